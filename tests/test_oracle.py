@@ -1,0 +1,267 @@
+"""Pins the C f64 oracle (oracle/so_oracle.c) before anything is compared to it.
+
+Checks, per oracle function:
+  * the crate's own known answers (tests/fft_tests.rs:49-77, :114-124), exactly;
+  * the crate's committed scenario fixtures (tests/test_data/*.npy, reduced copies
+    in tests/golden/reference_fixtures.npz) reproduced by the same operation stacks;
+  * library routines (numpy.fft, numpy leggauss, scipy eval_legendre), closed forms
+    (Strang-split harmonic oscillator, Trotterised imaginary-time ground energy),
+    invariants (unitarity, border loss bookkeeping) and brute force (dense DFT).
+"""
+import numpy as np
+import pytest
+from scipy.special import eval_legendre
+
+from oracle import oracle as O
+from tests import scenarios as S
+
+
+def rel(a, b):
+    return np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(np.asarray(b)))
+
+
+# --------------------------------------------------------------- known answers
+def test_fft_known_answer_and_exact_roundtrip(fixtures):
+    """tests/fft_tests.rs:12-78 — exact transform along axis 0 and bit-exact roundtrip."""
+    a = fixtures["fft4x4_input"]
+    g1 = O.Grid.new_linear_countable("a", 0.0, 1.0, 4, 0)
+    g2 = O.Grid.new_linear_countable("b", 0.0, 1.0, 4, 1)
+    wf = O.WaveFunction(a, [g1, g2])
+    fft = O.fft_transformation(g1, "a_fft")
+    fft.transform(wf)
+    transformed = wf.array.reshape(4, 4)
+    fft.inverse_transform(wf)
+    assert np.array_equal(transformed, fixtures["fft4x4_expected"])
+    assert np.array_equal(wf.array.reshape(4, 4), a)
+
+
+def test_fft_norm_roundtrip_exact(fixtures):
+    """tests/fft_tests.rs:81-126 — norm before equals norm after the roundtrip exactly."""
+    a = fixtures["fft4x4_input"]
+    g1 = O.Grid.new_linear_countable("a", 0.0, 1.0, 4, 0)
+    g2 = O.Grid.new_linear_countable("b", 0.0, 1.0, 4, 1)
+    wf = O.WaveFunction(a, [g1, g2])
+    fft = O.fft_transformation(g1, "a_fft")
+    n1 = wf.norm()
+    fft.transform(wf)
+    n2 = wf.norm()
+    fft.inverse_transform(wf)
+    n3 = wf.norm()
+    assert n1 == n3
+    # the momentum-space norm uses dk weights (halved at N/2-1 and N/2), so it differs
+    assert n2 != n1
+
+
+# --------------------------------------------------------------- crate fixtures
+@pytest.mark.parametrize("im_time,name", [(False, "harmonic"), (True, "harmonic_imaginary")])
+def test_harmonic_fixture(fixtures, im_time, name):
+    """tests/harmonic_tests.rs:9-31 via tests/harmonic_trap.rs:34-111."""
+    p, sv, V = S.harmonic_trap(O, im_time, name)
+    assert np.array_equal(V, fixtures[name + "_potential"])
+    p.propagate()
+    data, times = sv.frames(50)
+    assert np.array_equal(times, fixtures[name + "_time"])
+    cols = fixtures[name + "__cols"]
+    assert rel(data[:, cols], fixtures[name + "__frames"]) < 1e-12
+
+
+def test_lj_ground_fixture(fixtures):
+    """tests/ground_state_tests.rs:31-139 (imaginary time + mean_energy)."""
+    p, sv, V = S.lj_ground(O)
+    assert rel(V, fixtures["lj_ground_space_potential"]) < 1e-15
+    p.propagate()
+    data, times = sv.frames(120)
+    assert np.array_equal(times, fixtures["lj_ground_space_time"])
+    cols = fixtures["lj_ground_space__cols"]
+    assert rel(data[:, cols], fixtures["lj_ground_space__frames"]) < 1e-12
+    e = p.mean_energy()
+    assert e == pytest.approx(-2.10869e-4, rel=1e-5)
+
+
+@pytest.mark.parametrize("momentum_saver,name", [(False, "free_wave"), (True, "free_momentum")])
+def test_free_wave_fixture(fixtures, momentum_saver, name):
+    """tests/free_wave.rs:10-84. The quantum crate's unit constants are not in the
+    reference tree; with CODATA 2018 values the frames agree to ~2e-6, so the
+    tolerance here is loose (the harmonic and LJ fixtures pin the arithmetic)."""
+    p, sv = S.free_wave(O, momentum_saver=momentum_saver)
+    p.propagate()
+    data, times = sv.frames(50)
+    assert np.array_equal(times, fixtures[name + "_time"])
+    cols = fixtures[name + "__cols"]
+    assert rel(data[:, cols], fixtures[name + "__frames"]) < 1e-5
+    if momentum_saver:
+        k, _ = O.momentum_grid(O.Grid.new_linear_continuos("space", 0.0, 30.0, 512, 0))
+        assert rel(k, fixtures["free_momentum_momentum_grid"]) < 1e-15
+
+
+# --------------------------------------------------------------- library routines
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 6, 8, 12, 15, 64, 96, 256, 512, 768, 1024, 1536, 2048])
+def test_fft_raw_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert rel(O.fft_raw(x, -1), np.fft.fft(x)) < 1e-13
+    assert rel(O.fft_raw(x, +1), np.fft.ifft(x) * n) < 1e-13
+
+
+def test_fft_raw_matches_dense_dft():
+    n = 24
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    k = np.arange(n)
+    dft = np.exp(-2j * np.pi * np.outer(k, k) / n)
+    assert rel(O.fft_raw(x, -1), dft @ x) < 1e-13
+
+
+@pytest.mark.parametrize("n", [2, 5, 32, 64, 128])
+def test_gauss_legendre_matches_numpy(n):
+    x, w = O.gauss_legendre(n)
+    xr, wr = np.polynomial.legendre.leggauss(n)
+    assert np.max(np.abs(x - xr)) < 1e-14
+    assert np.max(np.abs(w - wr)) < 1e-13
+    # exact quadrature of x^(2n-2)
+    assert abs(np.sum(w * x ** (2 * n - 2)) - 2.0 / (2 * n - 1)) < 1e-13
+
+
+def test_legendre_polynomials_match_scipy():
+    for x in [-1.0, -0.3, 0.0, 0.77, 1.0]:
+        p = O.legendre_polynomials(60, x)
+        ref = np.array([eval_legendre(l, x) for l in range(61)])
+        assert np.max(np.abs(p - ref)) < 1e-13
+
+
+@pytest.mark.parametrize("n", [32, 64, 128])
+def test_legendre_matrix_orthonormal_on_gl_grid(n):
+    """legendre_diagonalization.rs:13-45 with GL nodes: T Tinv = I."""
+    x, w = O.gauss_legendre(n)
+    theta = np.arccos(x)[::-1].copy()
+    polar = O.Grid.new_custom("theta", theta, w[::-1].copy(), 1)
+    t, ti = O.legendre_matrices(polar)
+    assert np.all(t.imag == 0) and np.all(ti.imag == 0)
+    assert np.max(np.abs(t @ ti - np.eye(n))) < 1e-11
+    # Tinv[j, i] = sqrt(l_i + 1/2) P_l(cos theta_j)
+    l = np.arange(n)
+    ref = np.sqrt(l[None, :] + 0.5) * np.array([[eval_legendre(li, np.cos(th)) for li in l] for th in theta])
+    assert np.max(np.abs(ti.real - ref)) < 1e-12
+
+
+def test_matrix_transformation_matches_numpy():
+    """matrix_transformation.rs:58-75 along the middle axis of a 3-D array."""
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((3, 5, 4)) + 1j * rng.standard_normal((3, 5, 4))
+    m = rng.standard_normal((5, 5)) + 1j * rng.standard_normal((5, 5))
+    mi = np.linalg.inv(m)
+    gs = [O.Grid.new_linear_countable(n, 0.0, 1.0, s, d) for d, (n, s) in enumerate(zip("abc", a.shape))]
+    lg = O.Grid.new_linear_countable("b_t", 0.0, 1.0, 5, 1)
+    wf = O.WaveFunction(a, gs)
+    op = O.matrix_transformation(gs[1], lg, m, mi)
+    op.transform(wf)
+    assert rel(wf.array.reshape(a.shape), np.einsum("ij,ajc->aic", m, a)) < 1e-14
+    assert wf.grid(1).name == "b_t"
+    op.inverse_transform(wf)
+    assert rel(wf.array.reshape(a.shape), a) < 1e-13
+    assert wf.grid(1).name == "b"
+
+
+# --------------------------------------------------------------- closed forms
+def harmonic_1d(B, im_time, dt, steps, leak=True, n=1024, x0=2.0):
+    """Workload: 1-D harmonic trap, 1024 points on [-20, 20], V = x^2/2, hbar=m=omega=1."""
+    g = B.Grid.new_linear_continuos("x", -20.0, 20.0, n, 0)
+    x = g.nodes
+    arr = np.array([B.gaussian_distribution(xi, x0, 1.0, 0.0) for xi in x])
+    wf = B.WaveFunction(arr, [g])
+    wf.normalize(1.0)
+    p = B.Propagation()
+    p.set_wave_function(wf)
+    p.set_time_grid(dt, steps, im_time)
+    V = 0.5 * x * x
+    lc = B.leak_control()
+    lc.add_loss_checker("leak")
+    if leak:
+        p.add_control(lc, B.APPLY_FIRST | B.APPLY_SECOND)
+    p.add_propagator(B.one_dim_propagator(B.exp_propagator(V, B.select_step(False, dt, im_time)), 0))
+    p.add_transformation(B.fft_transformation(g, "k"))
+    p.add_propagator(B.one_dim_propagator(B.exp_propagator(B.kinetic_hamiltonian(g, 1.0), B.select_step(True, dt, im_time)), 0))
+    return p, g
+
+
+def test_strang_harmonic_ehrenfest_closed_form():
+    """Strang splitting of the harmonic oscillator is an exact leapfrog for <x>:
+    <x>_n = x0 cos(n theta), cos theta = 1 - (omega dt)^2 / 2 (1000 steps at dt = 0.01)."""
+    dt, steps = 0.01, 1000
+    p, g = harmonic_1d(O, False, dt, steps, leak=False)
+    p.propagate()
+    wf = p.wave_function()
+    x = g.nodes
+    rho = np.abs(wf.array) ** 2 * g.weights
+    mean_x = np.sum(rho * x) / np.sum(rho)
+    theta = np.arccos(1 - dt * dt / 2)
+    assert abs(mean_x - 2.0 * np.cos(steps * theta)) < 1e-10
+    # and the plain coherent-state centre differs at the O(dt^2) level
+    assert abs(mean_x - 2.0 * np.cos(steps * dt)) > 1e-6
+
+
+def test_imaginary_time_trotter_ground_energy():
+    """The Trotterised imaginary-time generator exp(-V dt/2) exp(-T dt) exp(-V dt/2) of
+    the harmonic oscillator has ground energy arccosh(1 + omega^2 dt^2 / 2) / (2 dt)
+    in the crate's decay estimator (propagation.rs:245-282)."""
+    dt = 0.01
+    p, _ = harmonic_1d(O, True, dt, 1500)
+    p.propagate()
+    e = p.mean_energy()
+    assert abs(e - np.arccosh(1 + dt * dt / 2) / (2 * dt)) < 1e-9
+
+
+def test_real_time_mean_energy_of_ground_state():
+    dt = 0.01
+    p, _ = harmonic_1d(O, True, dt, 1500)
+    p.propagate()
+    gs = p.wave_function().array
+    g = O.Grid.new_linear_continuos("x", -20.0, 20.0, 1024, 0)
+    q = O.Propagation()
+    wf = O.WaveFunction(gs, [g])
+    wf.normalize(1.0)
+    q.set_wave_function(wf)
+    q.set_time_grid(dt, 10, False)
+    V = 0.5 * g.nodes ** 2
+    q.add_propagator(O.one_dim_propagator(O.exp_propagator(V, O.select_step(False, dt, False)), 0))
+    q.add_transformation(O.fft_transformation(g, "k"))
+    q.add_propagator(O.one_dim_propagator(O.exp_propagator(O.kinetic_hamiltonian(g, 1.0), O.select_step(True, dt, False)), 0))
+    e = q.mean_energy()
+    assert abs(e - np.arccos(1 - dt * dt / 2) / (2 * dt)) < 1e-9
+
+
+# --------------------------------------------------------------- invariants
+def test_real_time_unitarity_and_time_reversal():
+    p, g = harmonic_1d(O, False, 0.01, 200, leak=False)
+    wf0 = p.wave_function().array.copy()
+    n0 = p.wave_function().norm()
+    p.propagate()
+    assert abs(p.wave_function().norm() - n0) < 1e-12
+    # a step at -dt undoes a step at dt
+    q, _ = harmonic_1d(O, False, -0.01, 200, leak=False)
+    q.set_wave_function(O.WaveFunction(p.wave_function().array, [g]))
+    q.propagate()
+    assert rel(q.wave_function().array, wf0) < 1e-11
+
+
+def test_border_loss_bookkeeping():
+    """border_dumping.rs:62-84: loss += norm_before - norm_after, and the loss of one
+    masked application equals sum (1 - mask^2) |psi|^2 w."""
+    g = O.Grid.new_linear_continuos("r", 0.0, 10.0, 200, 0)
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal(200) + 1j * rng.standard_normal(200)
+    wf = O.WaveFunction(a, [g])
+    mask = O.dumping_end(2.0, 0.5, g)
+    # dumping_end: 1 below r_max - width, 0 above r_max - end, sine ramp between
+    r = g.nodes
+    assert np.all(mask[r < 8.0] == 1) and np.all(mask[r > 9.5] == 0)
+    b = O.border_dumping(mask, g)
+    b.add_loss_checker("border")
+    p = O.Propagation()
+    p.set_wave_function(wf)
+    p.set_time_grid(1.0, 1, False)
+    p.add_control(b, O.APPLY_FIRST)
+    p.add_propagator(O.one_dim_propagator(np.ones(200), 0))
+    expected = np.sum((1 - np.abs(mask) ** 2) * np.abs(a) ** 2 * g.weights)
+    p.step_once()
+    assert abs(b.loss() - expected) < 1e-12 * np.sum(np.abs(a) ** 2 * g.weights)
