@@ -1,0 +1,142 @@
+"""Benchmark: 2-D radial x angular real-time split-operator propagation on one MI355X.
+
+Workload (the headline configuration): r in [0.5, 50] on 2048 points, 128
+Gauss-Legendre theta nodes, Legendre transform to l = 0..127, V(r, theta) =
+C12/r^12 - C6/r^6 (1 + a P2(cos theta)) with seeded (C6, C12, a), m = 1,
+Gaussian r0 = 8, sigma = 1 times P0, dt = 0.002, border damping on the last
+10% of r with its loss checked each step. Complex FP32 storage, FP64 norms.
+
+A step is one Propagation::step of that stack. The timed region is K steps
+with psi resident in HBM, bracketed by device synchronisation. Per-kernel
+durations come from HIP events on the library's stream (each pass of the
+steady-state step launched on its own, back to back). The CPU baseline is
+the f64 C oracle on the host cores for a bounded number of steps.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grid-point-steps/s; time-to-ground-state at 1e-8 E, learned vs plain, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK_TFLOPS = 157.3
+N_R, N_THETA, DT = 2048, 128, 0.002
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=20, help="steps of the CPU oracle baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-iters", type=int, default=200)
+    return ap.parse_args()
+
+
+def pass_record(t):
+    pts = t["points"]
+    cb = t["complex_bytes"]
+    # algorithmic bytes: psi read + written once, each full-grid operator table read once
+    bytes_ = (2 + t["nd_tables"]) * pts * cb
+    sec = t["avg_ms"] * 1e-3
+    kind = {0: "elementwise", 1: "fft", 2: "legendre_gemm"}[t["xform_type"]]
+    rec = {
+        "kernel": kind, "transforms": t["nx"], "axis": t["axis"], "avg_us": t["avg_ms"] * 1e3,
+        "workgroups": t["tiles"], "alg_bytes": bytes_, "gbs": bytes_ / sec / 1e9,
+        "mat_flops": t["mat_flops"], "fft_flops": t["fft_flops"],
+    }
+    if t["mat_flops"] > 0:
+        rec["tflops"] = t["mat_flops"] / sec / 1e12
+    return rec
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md and not launched", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import split_operator_amd as so
+    from split_operator_amd import workloads as W
+
+    seed = 1 * 1000 + 0
+    p, (radial, polar), (c6, c12, a) = W.radial_angular(
+        so, n_r=N_R, n_theta=N_THETA, dt=DT, steps=args.steps, seed=seed, dtype="f32",
+        gl=so.gauss_legendre(N_THETA))
+    p.steps(args.warmup)
+    so.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p.steps(args.steps)
+    so.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    pts = N_R * N_THETA
+    value = pts * args.steps / wall
+    stats = p.stats()
+
+    # accuracy of the FP32 run against the same stack stored in FP64 on the GPU
+    q, _, _ = W.radial_angular(so, n_r=N_R, n_theta=N_THETA, dt=DT, steps=args.steps, seed=seed, dtype="f64",
+                               gl=so.gauss_legendre(N_THETA))
+    q.steps(args.warmup + args.steps)
+    a32 = p.wave_function().array
+    a64 = q.wave_function().array
+    l2 = float(np.linalg.norm(a32 - a64) / np.linalg.norm(a64))
+    del q
+
+    # per-kernel timing of the steady-state step (overwrites p's state: done last)
+    passes = [pass_record(t) for t in p.time_passes(args.kernel_iters)]
+    dom = max(passes, key=lambda r: r["avg_us"])
+    if dom["kernel"] == "legendre_gemm":
+        achieved = dom["tflops"]
+        roof = {"bound": "mfma", "achieved": achieved, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP32_MFMA_PEAK_TFLOPS}
+    else:
+        roof = {"bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": dom["gbs"] / HBM_PEAK_GBS}
+    roof["kernel"] = dom["kernel"]
+    roof["traffic"] = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_radial_angular_2048x128.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            roof["traffic"] = json.load(f).get(dom["kernel"])
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        n_cpu = args.cpu_steps
+        po, _, _ = W.radial_angular(O, n_r=N_R, n_theta=N_THETA, dt=DT, steps=n_cpu, seed=seed)
+        po.steps(1)
+        t1 = time.perf_counter()
+        po.steps(n_cpu)
+        cw = time.perf_counter() - t1
+        cpu = {"value": pts * n_cpu / cw, "unit": "grid-point-steps/s", "cores": O.num_threads(), "kind": "port",
+               "sample": f"f64 C oracle, same 2048x128 stack, {n_cpu} steps after 1 warm-up step ({cw:.1f} s)"}
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "grid-point-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "radial_angular_realtime", "n_r": N_R, "n_theta": N_THETA, "dt": DT,
+                   "potential": {"C6": c6, "C12": c12, "a": a}, "storage": "complex f32, f64 norms",
+                   "border_damping": "last 10% of r, loss checked", "seed": seed},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "launches_per_step": stats["launches_per_step"],
+        "rel_l2_vs_f64_gpu": l2,
+        "passes": passes,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

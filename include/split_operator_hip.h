@@ -1,0 +1,212 @@
+/*
+ * split_operator_hip.h — C ABI of the MI355X (gfx950) split-operator
+ * propagation path.
+ *
+ * The entry points replace, one for one, the Rust crate `split_operator`
+ * (v0.1.0) types on the propagation hot path. Each declaration names the crate
+ * item it stands for (path:line in the crate). A Rust binding keeps the crate's
+ * Grid/TimeGrid/hamiltonian factories on the host and hands arrays across
+ * this boundary; the wave function lives on the GPU.
+ *
+ * Conventions
+ *   - every function returns 0 (SO_OK) or a negative SO_ERR_* code; the
+ *     message of the last failure on the calling thread is so_last_error();
+ *   - complex arrays are interleaved (re, im) f64, C order, as
+ *     ndarray::ArrayD<Complex64> stores them;
+ *   - host pointers are borrowed for the duration of the call only;
+ *   - `*_create` returns an owned handle; ops added to an operation stack and
+ *     stacks set on a propagation are moved (ownership passes, do not destroy);
+ *   - a handle is used by one host thread at a time; a context owns one HIP
+ *     stream and every call on its handles is ordered on it.
+ */
+#ifndef SPLIT_OPERATOR_HIP_H
+#define SPLIT_OPERATOR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SO_OK = 0,
+    SO_ERR_INVALID = -1,     /* null handle / bad argument */
+    SO_ERR_SHAPE = -2,       /* shape mismatch (the crate's assert!s) */
+    SO_ERR_UNSUPPORTED = -3, /* e.g. odd or non 2^a 3^b FFT length, ndim > 4 */
+    SO_ERR_STACK = -4,       /* mean_energy precondition (propagation.rs:249-262) */
+    SO_ERR_HIP = -5,         /* HIP runtime error */
+    SO_ERR_NONFINITE = -6,   /* a norm reduction produced NaN/Inf */
+    SO_ERR_IO = -7,          /* saver file write failed (Saver::save Err) */
+    SO_ERR_STATE = -8        /* e.g. propagation without a wave function */
+};
+
+enum { SO_COMPLEX_F32 = 0, SO_COMPLEX_F64 = 1 };                /* device storage of psi */
+enum { SO_APPLY_NONE = 0, SO_APPLY_FIRST_HALF = 1, SO_APPLY_SECOND_HALF = 2 }; /* control.rs:19-26 */
+enum { SO_ORDER_NORMAL = 0, SO_ORDER_INVERSE_FIRST = 1 };        /* transformation.rs:13-17 */
+
+typedef struct so_context so_context;
+typedef struct so_wave_function so_wave_function;
+typedef struct so_op so_op;
+typedef struct so_operation_stack so_operation_stack;
+typedef struct so_propagation so_propagation;
+
+const char *so_last_error(void);
+
+/* ---- context (no crate counterpart: device + stream + scalar arena) ---- */
+int so_context_create(int device, so_context **out);
+int so_context_destroy(so_context *ctx);
+int so_context_synchronize(so_context *ctx);
+/* the context's hipStream_t (for events and interop) */
+int so_context_stream(so_context *ctx, void **stream);
+
+/* ---- WaveFunction — src/wave_function.rs:10-154, Grid src/grid.rs:13-89 ---- */
+/* WaveFunction::new (wave_function.rs:23-47): shape + storage dtype; then one
+ * so_wave_function_set_grid per axis and an upload of the array */
+int so_wave_function_create(so_context *ctx, int ndim, const int64_t *shape, int dtype, so_wave_function **out);
+/* the Grid of axis `axis` (grid.rs:13-20: name, nodes, weights) */
+int so_wave_function_set_grid(so_wave_function *wf, int axis, const char *name, int64_t n, const double *nodes,
+                              const double *weights);
+/* current grid of an axis (after transformations swapped it): name into buf
+ * (cap bytes), n, nodes/weights copied when non-null (n doubles each) */
+int so_wave_function_get_grid(so_wave_function *wf, int axis, char *name, int64_t cap, int64_t *n, double *nodes,
+                              double *weights);
+/* WaveFunction.array (pub field, wave_function.rs:12) write / read */
+int so_wave_function_upload(so_wave_function *wf, const double *data);
+int so_wave_function_download(so_wave_function *wf, double *data);
+int so_wave_function_norm(so_wave_function *wf, double *out);                 /* wave_function.rs:68-85 */
+int so_wave_function_normalize(so_wave_function *wf, double new_norm);        /* wave_function.rs:104-109 */
+int so_wave_function_dot(so_wave_function *a, so_wave_function *b, double *out_re_im); /* wave_function.rs:87-101 */
+int so_wave_function_density(so_wave_function *wf, double *out);              /* wave_function.rs:112-118 */
+int so_wave_function_state_density(so_wave_function *wf, int axis, double *out); /* wave_function.rs:121-148 */
+int so_wave_function_clone(so_wave_function *wf, so_wave_function **out);     /* #[derive(Clone)] :10 */
+int so_wave_function_destroy(so_wave_function *wf);
+/* WaveFunction.array.shape(): *ndim and up to 4 extents into shape */
+int so_wave_function_shape(so_wave_function *wf, int *ndim, int64_t *shape);
+/* device address of psi (complex f32/f64, C order) for zero-copy interop */
+int so_wave_function_device_ptr(so_wave_function *wf, void **ptr);
+
+/* ---- Propagators — src/propagator.rs:12-18 ---- */
+/* OneDimPropagator::new + set_operator (one_dim_propagator.rs:17-29): op has n complex entries */
+int so_one_dim_propagator_create(so_context *ctx, int dimension_no, int64_t n, const double *op, so_op **out);
+int so_one_dim_propagator_set_operator(so_op *p, int64_t n, const double *op);  /* :25-29 */
+int so_one_dim_propagator_add_operator(so_op *p, int64_t n, const double *op);  /* :31-35 */
+/* NDimPropagator::new (n_dim_propagator.rs:15-20; holds zeros([1]) until set) */
+int so_n_dim_propagator_create(so_context *ctx, so_op **out);
+int so_n_dim_propagator_set_operator(so_op *p, int ndim, const int64_t *shape, const double *op); /* :22-24 */
+int so_n_dim_propagator_add_operator(so_op *p, int ndim, const int64_t *shape, const double *op); /* :26-30 */
+/* NonDiagPropagator (non_diagonal_propagator.rs:16-27): `lanes` complex n x n matrices, applied to the lanes
+ * along dimension_no in C order */
+int so_non_diag_propagator_create(so_context *ctx, int dimension_no, int64_t lanes, int64_t n, const double *ops,
+                                  so_op **out);
+/* set_loss_checked (one_dim_propagator.rs:48-50, n_dim_propagator.rs:38-40) with
+ * LossChecker::new (loss_checker.rs:16-23) when frames_no == 0, else
+ * LossChecker::new_with_saver (loss_checker.rs:25-37) writing "<filename>.dat" */
+int so_propagator_set_loss_checked(so_op *p, const char *name, int64_t frames_no, const char *filename, double step,
+                                   int64_t step_no);
+
+/* ---- Transformations — src/propagator/transformation.rs:5-17 ---- */
+/* FFTTransformation::new (fft_transformation.rs:24-48): grid nodes along dimension_no */
+int so_fft_transformation_create(so_context *ctx, int dimension_no, int64_t n, const double *nodes,
+                                 const char *transformed_grid_name, so_op **out);
+/* MatrixTransformation::new (matrix_transformation.rs:21-33) with its target grid */
+int so_matrix_transformation_create(so_context *ctx, int dimension_no, int64_t n, const char *grid_name,
+                                    const double *grid_nodes, const double *grid_weights, so_op **out);
+/* set_diagonalization_matrix (matrix_transformation.rs:35-51): n x n complex, row-major */
+int so_matrix_transformation_set_matrix(so_op *t, int64_t n, const double *transformation,
+                                        const double *inverse_transformation);
+/* the held grid_transformation (pub field, fft_transformation.rs:19, matrix_transformation.rs:16) */
+int so_transformation_grid(so_op *t, char *name, int64_t cap, int64_t *n, double *nodes, double *weights);
+/* Transformation::transform / inverse_transform applied directly to a wave function */
+int so_transformation_transform(so_op *t, so_wave_function *wf);
+int so_transformation_inverse_transform(so_op *t, so_wave_function *wf);
+/* Propagator::apply directly on a wave function (propagator.rs:13) */
+int so_propagator_apply(so_op *p, so_wave_function *wf);
+
+/* ---- Controls — src/control.rs:6-17 ---- */
+/* BorderDumping::new (border_dumping.rs:42-50): mask of n complex entries along dimension_no */
+int so_border_dumping_create(so_context *ctx, int dimension_no, int64_t n, const double *mask, so_op **out);
+int so_leak_control_create(so_context *ctx, so_op **out);          /* leak_control.rs:14-19 */
+/* add_loss_checker (border_dumping.rs:52-54, leak_control.rs:21-23) */
+int so_control_add_loss_checker(so_op *c, const char *name, int64_t frames_no, const char *filename, double step,
+                                int64_t step_no);
+int so_control_name(so_op *c, char *name, int64_t cap);            /* Control::name */
+/* LossChecker::loss (loss_checker.rs:40-46, writes the LossSaver file when present);
+ * *present = 0 when the op has no checker */
+int so_op_loss(so_op *op, double *out, int *present);
+/* LossSaver records (loss_saver.rs:29-39): up to cap (time, loss) pairs */
+int so_op_loss_frames(so_op *op, double *times, double *losses, int64_t cap, int64_t *n);
+
+/* ---- Savers — src/saver.rs:5-14 ---- */
+/* StateSaver::new (wave_function_saver.rs:118-133): state grid (name, nodes) on dimension_no */
+int so_state_saver_create(so_context *ctx, const char *name, double step, int64_t step_no, int dimension_no,
+                          const char *grid_name, int64_t n, const double *nodes, int64_t frames_no, so_op **out);
+/* WaveFunctionSaver::new (wave_function_saver.rs:21-38) */
+int so_wave_function_saver_create(so_context *ctx, const char *name, double step, int64_t step_no, int64_t nx,
+                                  const double *x_nodes, int64_t ny, const double *y_nodes, int64_t frames_no,
+                                  so_op **out);
+int so_saver_save(so_op *s);    /* Saver::save — writes "<name>.npy", grid and time .npy files */
+int so_saver_reset(so_op *s);   /* Saver::reset */
+/* recorded frames: data (per_frame x frames_no, C order as saved) and times */
+int so_saver_frames(so_op *s, double *data, double *times, int64_t *n_times);
+
+int so_op_destroy(so_op *op);   /* only for ops never added to a stack */
+
+/* ---- OperationStack — src/propagation.rs:26-65 ---- */
+int so_operation_stack_create(so_operation_stack **out);
+int so_operation_stack_add_propagator(so_operation_stack *s, so_op *p);                   /* :43-45 */
+int so_operation_stack_add_transformation(so_operation_stack *s, so_op *t, int order);    /* :49-51 */
+int so_operation_stack_add_saver(so_operation_stack *s, so_op *sv, int apply);           /* :55-59 */
+int so_operation_stack_add_control(so_operation_stack *s, so_op *c, int apply);          /* :62-64 */
+int so_operation_stack_len(so_operation_stack *s, int64_t *out);                         /* :38-40 */
+int so_operation_stack_destroy(so_operation_stack *s);
+
+/* ---- Propagation — src/propagation.rs:74-292 ---- */
+int so_propagation_create(so_context *ctx, so_propagation **out);                        /* Default */
+int so_propagation_set_wave_function(so_propagation *p, so_wave_function *wf);           /* :91-93 (copies) */
+int so_propagation_set_time_grid(so_propagation *p, double step, int64_t step_no, int im_time); /* :96-98 */
+int so_propagation_set_operation_stack(so_propagation *p, so_operation_stack *s);        /* :101-103 (moves) */
+int so_propagation_wave_function(so_propagation *p, so_wave_function **borrowed);        /* :111-113 */
+int so_propagation_propagate(so_propagation *p);                                         /* :185-190 */
+/* n steps of Propagation::step (:134-182), which the crate keeps private */
+int so_propagation_step(so_propagation *p, int64_t n);
+int so_propagation_mean_energy(so_propagation *p, double *out);                          /* :245-291 */
+int so_propagation_get_losses(so_propagation *p, double *out, int64_t cap, int64_t *n);  /* :216-231 */
+int so_propagation_print_losses(so_propagation *p);                                      /* :193-213 */
+int so_propagation_savers_save(so_propagation *p);                                       /* :234-243 */
+int so_propagation_reset_savers_state(so_propagation *p);                                /* :116-122 */
+int so_propagation_reset_losses(so_propagation *p);                                      /* :125-131 */
+/* execution options: hipGraph replay of steady-state steps (default on) */
+int so_propagation_set_graphs(so_propagation *p, int enable);
+/* kernels launched per step in the last steady state and passes compiled */
+int so_propagation_stats(so_propagation *p, int64_t *launches_per_step, int64_t *compiled_passes);
+/* per-pass timing of the last steady-state step (benchmark use: overwrites the
+ * propagation's state). One record per kernel pass. */
+typedef struct {
+    double avg_ms;        /* mean duration of one launch */
+    int64_t points;       /* grid points of the wave function */
+    int64_t lane;         /* lane length along the pass axis */
+    int64_t lanes;        /* number of lanes */
+    int64_t tiles;        /* workgroups per launch */
+    double mat_flops;     /* algorithmic GEMM flops per launch (4 n^2 per lane, 8 n^2 complex matrix) */
+    double fft_flops;     /* 5 n log2 n per lane per FFT */
+    int32_t complex_bytes;/* 8 (complex f32) or 16 */
+    int32_t nx;           /* lane transforms in the pass */
+    int32_t xform_type;   /* 0 none, 1 FFT, 2 matrix */
+    int32_t axis;
+    int32_t lanes_per_tile;
+    int32_t nd_tables;    /* full-grid operator tables read */
+    int32_t norms;        /* FP64 norm reductions */
+    int32_t line_ops;     /* one-dimensional tables and scale factors */
+} so_pass_timing;
+int so_propagation_time_passes(so_propagation *p, int64_t iters, int64_t cap, so_pass_timing *out, int64_t *n);
+int so_propagation_destroy(so_propagation *p);
+
+/* ---- helpers without a crate counterpart ---- */
+/* Gauss-Legendre nodes (ascending) and weights computed in FP64 on the GPU;
+ * the crate's callers supply the polar grid of legendre_diagonalization.rs:13 */
+int so_gauss_legendre(so_context *ctx, int64_t n, double *x, double *w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

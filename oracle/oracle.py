@@ -104,6 +104,7 @@ def lib():
             "orc_prop_reset_savers_state": (None, [vp]),
             "orc_prop_wave_function": (vp, [vp]),
             "orc_fft_raw": (None, [_dp, _dp, C.c_int64, C.c_int]),
+            "orc_num_threads": (C.c_int, []),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -111,6 +112,10 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def num_threads():
+    return lib().orc_num_threads()
 
 
 def _d(a):

@@ -13,6 +13,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define ORC_MAXDIM 8
 
@@ -78,6 +81,14 @@ static orc_c c_exp(orc_c z) {
     double e = exp(re);
     orc_c r = {e * cos(im), e * sin(im)};
     return r;
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 /* ======================================================================

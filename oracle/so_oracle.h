@@ -19,6 +19,9 @@ extern "C" {
 
 typedef struct { double re, im; } orc_c;
 
+/* threads the lane-parallel loops use (OpenMP) */
+int orc_num_threads(void);
+
 /* ---- Grid (src/grid.rs:13-89) ---------------------------------------- */
 typedef struct orc_grid {
     char name[64];

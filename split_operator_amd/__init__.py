@@ -1,0 +1,549 @@
+"""split_operator_amd — MI355X (gfx950) split-operator propagation behind the
+split_operator crate's API.
+
+Python mirror of the crate's public types over the C ABI of
+include/split_operator_hip.h: Grid (src/grid.rs), WaveFunction
+(src/wave_function.rs), the Propagator / Transformation / Control / Saver
+plugins (src/propagator.rs, src/propagator/transformation.rs, src/control.rs,
+src/saver.rs), OperationStack and Propagation (src/propagation.rs). Wave
+functions live on the GPU; every step runs in hand-written HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import SoError, call, dptr, i64ptr
+from .hamiltonian_factory import (dispersion, dumping_end, exp_propagator, exp_propagator_complex,  # noqa: F401
+                                  gaussian_distribution, harmonic, kinetic_hamiltonian, legendre_matrices,
+                                  legendre_polynomials, lennard_jones, momentum_grid, rotational_hamiltonian,
+                                  select_step, two_dim_into_n_dim_operator)
+
+APPLY_NONE = 0
+APPLY_FIRST = 1
+APPLY_SECOND = 2
+ORDER_NORMAL = 0
+ORDER_INVERSE_FIRST = 1
+COMPLEX_F32 = 0
+COMPLEX_F64 = 1
+
+_DTYPES = {"f32": COMPLEX_F32, "f64": COMPLEX_F64, "complex64": COMPLEX_F32, "complex128": COMPLEX_F64,
+           COMPLEX_F32: COMPLEX_F32, COMPLEX_F64: COMPLEX_F64}
+_default_dtype = COMPLEX_F64
+_ctx = None
+
+
+def set_default_dtype(dtype):
+    """Storage of new wave functions: "f64" (default, the crate's Complex64) or "f32"."""
+    global _default_dtype
+    _default_dtype = _DTYPES[dtype]
+
+
+def context():
+    """The process-wide context on device 0 (one HIP stream)."""
+    global _ctx
+    if _ctx is None:
+        h = C.c_void_p()
+        call("so_context_create", 0, C.byref(h))
+        _ctx = h
+    return _ctx
+
+
+def synchronize():
+    call("so_context_synchronize", context())
+
+
+def stream_handle():
+    """hipStream_t (as an int) the library launches on."""
+    p = C.c_void_p()
+    call("so_context_stream", context(), C.byref(p))
+    return p.value
+
+
+def _c128(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+# --------------------------------------------------------------------- Grid
+class Grid:
+    """src/grid.rs:13-89 (host object, as in the crate)."""
+
+    def __init__(self, name, dimension_no, nodes, weights):
+        self.name = name
+        self.dimension_no = dimension_no
+        self.nodes = _f64(nodes)
+        self.weights = _f64(weights)
+
+    @property
+    def nodes_no(self):
+        return len(self.nodes)
+
+    @staticmethod
+    def new_linear_continuos(name, start, end, nodes_no, dimension_no):
+        step = (end - start) / (nodes_no - 1.0)
+        nodes = start + step * np.arange(nodes_no, dtype=np.float64)
+        w = np.full(nodes_no, 1.0 * step)
+        w[0] = 0.5 * step
+        w[-1] = 0.5 * step
+        return Grid(name, dimension_no, nodes, w)
+
+    @staticmethod
+    def new_linear_countable(name, start, end, nodes_no, dimension_no):
+        step = (end - start) / (nodes_no - 1.0)
+        nodes = start + step * np.arange(nodes_no, dtype=np.float64)
+        return Grid(name, dimension_no, nodes, np.full(nodes_no, 1.0 * step))
+
+    @staticmethod
+    def new_custom(name, nodes, weights, dimension_no):
+        return Grid(name, dimension_no, nodes, weights)
+
+
+def gauss_legendre(n):
+    """Gauss-Legendre nodes (ascending) and weights, computed in FP64 on the GPU."""
+    x = np.zeros(n)
+    w = np.zeros(n)
+    call("so_gauss_legendre", context(), n, dptr(x), dptr(w))
+    return x, w
+
+
+# --------------------------------------------------------------------- WaveFunction
+class WaveFunction:
+    """src/wave_function.rs:10-154 with the array resident on the GPU."""
+
+    def __init__(self, array, grids, dtype=None, _handle=None, _owned=True):
+        if _handle is not None:
+            self._h = _handle
+            self._owned = _owned
+            self._read_shape()
+            return
+        a = _c128(array)
+        dt = _default_dtype if dtype is None else _DTYPES[dtype]
+        shape = np.array(a.shape, dtype=np.int64)
+        h = C.c_void_p()
+        call("so_wave_function_create", context(), a.ndim, i64ptr(shape), dt, C.byref(h))
+        self._h = h
+        self._owned = True
+        self.shape = a.shape
+        for ax, g in enumerate(grids):
+            call("so_wave_function_set_grid", h, ax, g.name.encode(), g.nodes_no, dptr(g.nodes), dptr(g.weights))
+        call("so_wave_function_upload", h, dptr(a.view(np.float64)))
+
+    def _read_shape(self):
+        nd = C.c_int()
+        shape = np.zeros(4, dtype=np.int64)
+        call("so_wave_function_shape", self._h, C.byref(nd), i64ptr(shape))
+        self.shape = tuple(int(x) for x in shape[: nd.value])
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape))
+
+    @property
+    def array(self):
+        out = np.zeros(self.size, dtype=np.complex128)
+        call("so_wave_function_download", self._h, dptr(out.view(np.float64)))
+        return out.reshape(self.shape)
+
+    @array.setter
+    def array(self, value):
+        a = _c128(value).reshape(self.shape)
+        call("so_wave_function_upload", self._h, dptr(a.view(np.float64)))
+
+    def grid(self, axis):
+        buf = C.create_string_buffer(256)
+        n = C.c_int64()
+        call("so_wave_function_get_grid", self._h, axis, buf, 256, C.byref(n), None, None)
+        nodes = np.zeros(n.value)
+        w = np.zeros(n.value)
+        call("so_wave_function_get_grid", self._h, axis, None, 0, C.byref(n), dptr(nodes), dptr(w))
+        return Grid(buf.value.decode(), axis, nodes, w)
+
+    @property
+    def grids(self):
+        return [self.grid(a) for a in range(len(self.shape))]
+
+    def norm(self):
+        out = C.c_double()
+        call("so_wave_function_norm", self._h, C.byref(out))
+        return out.value
+
+    def normalize(self, new_norm):
+        call("so_wave_function_normalize", self._h, float(new_norm))
+
+    def dot(self, other):
+        out = np.zeros(2)
+        call("so_wave_function_dot", self._h, other._h, dptr(out))
+        return complex(out[0], out[1])
+
+    def density(self):
+        out = np.zeros(self.size)
+        call("so_wave_function_density", self._h, dptr(out))
+        return out.reshape(self.shape)
+
+    def state_density(self, axis, n_axis=None):
+        n = self.size if len(self.shape) == 1 else self.shape[axis]
+        out = np.zeros(max(n, self.size))
+        call("so_wave_function_state_density", self._h, axis, dptr(out))
+        return out[:n]
+
+    def clone(self):
+        h = C.c_void_p()
+        call("so_wave_function_clone", self._h, C.byref(h))
+        return WaveFunction(None, None, _handle=h)
+
+    def device_ptr(self):
+        p = C.c_void_p()
+        call("so_wave_function_device_ptr", self._h, C.byref(p))
+        return p.value
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.so_wave_function_destroy(self._h)
+            self._h = None
+
+
+# --------------------------------------------------------------------- operations
+class Op:
+    """An operation handle (Propagator, Transformation, Control or Saver)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self._owned = True
+
+    def set_loss_checked(self, name, frames_no=0, step=0.0, step_no=0, filename=None):
+        """LossChecker::new / new_with_saver (loss_checker.rs:16-37)."""
+        fn = (filename or name).encode()
+        fname = "so_control_add_loss_checker" if self._is_control else "so_propagator_set_loss_checked"
+        call(fname, self._h, name.encode(), frames_no, fn, step, step_no)
+
+    add_loss_checker = set_loss_checked
+    _is_control = False
+
+    def loss(self):
+        out = C.c_double()
+        present = C.c_int()
+        call("so_op_loss", self._h, C.byref(out), C.byref(present))
+        return out.value if present.value else None
+
+    def loss_frames(self):
+        n = C.c_int64()
+        call("so_op_loss_frames", self._h, None, None, 0, C.byref(n))
+        t = np.zeros(max(n.value, 1))
+        l = np.zeros(max(n.value, 1))
+        call("so_op_loss_frames", self._h, dptr(t), dptr(l), n.value, C.byref(n))
+        return t[: n.value], l[: n.value]
+
+    def held_grid(self):
+        """The grid_transformation a transformation holds (swapped on transform)."""
+        buf = C.create_string_buffer(256)
+        n = C.c_int64()
+        call("so_transformation_grid", self._h, buf, 256, C.byref(n), None, None)
+        nodes = np.zeros(n.value)
+        w = np.zeros(n.value)
+        call("so_transformation_grid", self._h, None, 0, C.byref(n), dptr(nodes), dptr(w))
+        return Grid(buf.value.decode(), getattr(self, "dimension_no", 0), nodes, w)
+
+    grid_transformation = property(held_grid)
+
+    def apply(self, wf):
+        call("so_propagator_apply", self._h, wf._h)
+
+    def transform(self, wf):
+        call("so_transformation_transform", self._h, wf._h)
+
+    def inverse_transform(self, wf):
+        call("so_transformation_inverse_transform", self._h, wf._h)
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.so_op_destroy(self._h)
+            self._h = None
+
+
+class Control(Op):
+    _is_control = True
+
+    def name(self):
+        buf = C.create_string_buffer(64)
+        call("so_control_name", self._h, buf, 64)
+        return buf.value.decode()
+
+
+def one_dim_propagator(op, dimension_no):
+    """OneDimPropagator::new + set_operator (one_dim_propagator.rs:17-29)."""
+    op = _c128(op)
+    h = C.c_void_p()
+    call("so_one_dim_propagator_create", context(), dimension_no, op.size, dptr(op.view(np.float64)), C.byref(h))
+    o = Op(h)
+    o.dimension_no = dimension_no
+    return o
+
+
+def n_dim_propagator(op=None):
+    """NDimPropagator::new (+ set_operator) (n_dim_propagator.rs:15-24)."""
+    h = C.c_void_p()
+    call("so_n_dim_propagator_create", context(), C.byref(h))
+    o = Op(h)
+    if op is not None:
+        a = _c128(op)
+        shape = np.array(a.shape, dtype=np.int64)
+        call("so_n_dim_propagator_set_operator", h, a.ndim, i64ptr(shape), dptr(a.view(np.float64)))
+    return o
+
+
+def fft_transformation(grid, transformed_grid_name):
+    """FFTTransformation::new (fft_transformation.rs:24-48)."""
+    h = C.c_void_p()
+    call("so_fft_transformation_create", context(), grid.dimension_no, grid.nodes_no, dptr(grid.nodes),
+         transformed_grid_name.encode(), C.byref(h))
+    o = Op(h)
+    o.dimension_no = grid.dimension_no
+    return o
+
+
+def matrix_transformation(grid, grid_transformation, transformation=None, inverse_transformation=None):
+    """MatrixTransformation::new + set_diagonalization_matrix (matrix_transformation.rs:21-51)."""
+    h = C.c_void_p()
+    g = grid_transformation
+    call("so_matrix_transformation_create", context(), grid.dimension_no, grid.nodes_no, g.name.encode(),
+         dptr(g.nodes), dptr(g.weights), C.byref(h))
+    o = Op(h)
+    o.dimension_no = grid.dimension_no
+    if transformation is not None:
+        t = _c128(transformation)
+        ti = _c128(inverse_transformation)
+        call("so_matrix_transformation_set_matrix", h, grid.nodes_no, dptr(t.view(np.float64)),
+             dptr(ti.view(np.float64)))
+    return o
+
+
+def legendre_diagonalization_operator(polar_grid):
+    """legendre_diagonalization.rs:13-45: the angular_momentum grid and the Legendre matrices."""
+    n = polar_grid.nodes_no
+    l_grid = Grid.new_linear_countable("angular_momentum", 0.0, float(n - 1), n, polar_grid.dimension_no)
+    t, ti = legendre_matrices(polar_grid)
+    return matrix_transformation(polar_grid, l_grid, t, ti)
+
+
+def border_dumping(mask, grid):
+    """BorderDumping::new (border_dumping.rs:42-50)."""
+    m = _c128(mask)
+    h = C.c_void_p()
+    call("so_border_dumping_create", context(), grid.dimension_no, m.size, dptr(m.view(np.float64)), C.byref(h))
+    return Control(h)
+
+
+def leak_control():
+    """LeakControl::new (leak_control.rs:14-19)."""
+    h = C.c_void_p()
+    call("so_leak_control_create", context(), C.byref(h))
+    return Control(h)
+
+
+class Saver(Op):
+    def __init__(self, handle, per_frame, frames_no):
+        super().__init__(handle)
+        self.per_frame = per_frame
+        self.frames_no = frames_no
+
+    def frames(self, frames_no=None):
+        fn = frames_no or self.frames_no
+        data = np.zeros(self.per_frame * fn)
+        times = np.zeros(fn + 1)
+        n = C.c_int64()
+        call("so_saver_frames", self._h, dptr(data), dptr(times), C.byref(n))
+        return data.reshape(-1, fn), times[: n.value]
+
+    def save(self):
+        call("so_saver_save", self._h)
+
+    def reset(self):
+        call("so_saver_reset", self._h)
+
+
+def state_saver(name, step, step_no, state_grid, frames_no):
+    """StateSaver::new (wave_function_saver.rs:118-133)."""
+    h = C.c_void_p()
+    call("so_state_saver_create", context(), name.encode(), step, step_no, state_grid.dimension_no,
+         state_grid.name.encode(), state_grid.nodes_no, dptr(state_grid.nodes), frames_no, C.byref(h))
+    return Saver(h, state_grid.nodes_no, frames_no)
+
+
+def wave_function_saver(name, step, step_no, x_grid, y_grid, frames_no):
+    """WaveFunctionSaver::new (wave_function_saver.rs:21-38)."""
+    h = C.c_void_p()
+    call("so_wave_function_saver_create", context(), name.encode(), step, step_no, x_grid.nodes_no,
+         dptr(x_grid.nodes), y_grid.nodes_no, dptr(y_grid.nodes), frames_no, C.byref(h))
+    return Saver(h, x_grid.nodes_no * y_grid.nodes_no, frames_no)
+
+
+# --------------------------------------------------------------------- OperationStack / Propagation
+class OperationStack:
+    """src/propagation.rs:26-65. Ops added here are moved into the stack."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        call("so_operation_stack_create", C.byref(h))
+        self._h = h
+        self._ops = []
+
+    def _take(self, op):
+        op._owned = False
+        self._ops.append(op)
+
+    def add_propagator(self, op):
+        call("so_operation_stack_add_propagator", self._h, op._h)
+        self._take(op)
+
+    def add_transformation(self, op, order=ORDER_NORMAL):
+        if order is True:
+            order = ORDER_INVERSE_FIRST
+        elif order is False:
+            order = ORDER_NORMAL
+        call("so_operation_stack_add_transformation", self._h, op._h, order)
+        self._take(op)
+
+    def add_saver(self, op, apply):
+        call("so_operation_stack_add_saver", self._h, op._h, apply)
+        self._take(op)
+
+    def add_control(self, op, apply):
+        call("so_operation_stack_add_control", self._h, op._h, apply)
+        self._take(op)
+
+    def operations_len(self):
+        n = C.c_int64()
+        call("so_operation_stack_len", self._h, C.byref(n))
+        return n.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.so_operation_stack_destroy(self._h)
+            self._h = None
+
+
+class Propagation:
+    """src/propagation.rs:74-292. `add_*` build an implicit OperationStack that is
+    handed to the propagation before the first step."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        call("so_propagation_create", context(), C.byref(h))
+        self._h = h
+        self._stack = None
+        self._pending = None
+        self.step = 0.0
+
+    def set_wave_function(self, wf):
+        call("so_propagation_set_wave_function", self._h, wf._h)
+
+    def set_time_grid(self, step, step_no, im_time):
+        self.step = step
+        call("so_propagation_set_time_grid", self._h, step, step_no, 1 if im_time else 0)
+
+    def set_operation_stack(self, stack):
+        call("so_propagation_set_operation_stack", self._h, stack._h)
+        stack._h = None  # moved
+        self._stack = stack
+
+    def _pend(self):
+        if self._pending is None:
+            self._pending = OperationStack()
+        return self._pending
+
+    def add_propagator(self, op):
+        self._pend().add_propagator(op)
+
+    def add_transformation(self, op, inverse_first=False):
+        self._pend().add_transformation(op, ORDER_INVERSE_FIRST if inverse_first else ORDER_NORMAL)
+
+    def add_saver(self, op, apply):
+        self._pend().add_saver(op, apply)
+
+    def add_control(self, op, apply):
+        self._pend().add_control(op, apply)
+
+    def _ready(self):
+        if self._pending is not None:
+            self.set_operation_stack(self._pending)
+            self._pending = None
+
+    def step_once(self):
+        self.steps(1)
+
+    def steps(self, n):
+        self._ready()
+        call("so_propagation_step", self._h, n)
+
+    def propagate(self):
+        self._ready()
+        call("so_propagation_propagate", self._h)
+
+    def mean_energy(self):
+        self._ready()
+        out = C.c_double()
+        call("so_propagation_mean_energy", self._h, C.byref(out))
+        return out.value
+
+    def get_losses(self):
+        self._ready()
+        n = C.c_int64()
+        call("so_propagation_get_losses", self._h, None, 0, C.byref(n))
+        out = np.zeros(max(n.value, 1))
+        call("so_propagation_get_losses", self._h, dptr(out), n.value, C.byref(n))
+        return list(out[: n.value])
+
+    def print_losses(self):
+        self._ready()
+        call("so_propagation_print_losses", self._h)
+
+    def savers_save(self):
+        self._ready()
+        call("so_propagation_savers_save", self._h)
+
+    def reset_losses(self):
+        self._ready()
+        call("so_propagation_reset_losses", self._h)
+
+    def reset_savers_state(self):
+        self._ready()
+        call("so_propagation_reset_savers_state", self._h)
+
+    def set_graphs(self, enable):
+        call("so_propagation_set_graphs", self._h, 1 if enable else 0)
+
+    def stats(self):
+        a = C.c_int64()
+        b = C.c_int64()
+        call("so_propagation_stats", self._h, C.byref(a), C.byref(b))
+        return {"launches_per_step": a.value, "compiled_passes": b.value}
+
+    def time_passes(self, iters=50):
+        """Per-pass launch timing of the last steady-state step (overwrites the state)."""
+        n = C.c_int64()
+        recs = (_lib.PassTiming * 64)()
+        call("so_propagation_time_passes", self._h, iters, 64, C.cast(recs, C.c_void_p), C.byref(n))
+        out = []
+        for i in range(n.value):
+            r = recs[i]
+            out.append({f: getattr(r, f) for f, _ in _lib.PassTiming._fields_})
+        return out
+
+    def wave_function(self):
+        h = C.c_void_p()
+        call("so_propagation_wave_function", self._h, C.byref(h))
+        return WaveFunction(None, None, _handle=h, _owned=False)
+
+    def synchronize(self):
+        synchronize()
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.so_propagation_destroy(self._h)
+            self._h = None

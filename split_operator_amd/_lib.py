@@ -1,0 +1,152 @@
+"""ctypes loader for libsplit_operator_hip.so (the C ABI of include/split_operator_hip.h).
+
+The library is built in-tree by split_operator_amd/build.py. Loading fails
+loudly when it is missing: there is no CPU fallback on the product path.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libsplit_operator_hip.so")
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_vp = C.c_void_p
+_vpp = C.POINTER(C.c_void_p)
+
+ERRORS = {
+    -1: "SO_ERR_INVALID", -2: "SO_ERR_SHAPE", -3: "SO_ERR_UNSUPPORTED", -4: "SO_ERR_STACK",
+    -5: "SO_ERR_HIP", -6: "SO_ERR_NONFINITE", -7: "SO_ERR_IO", -8: "SO_ERR_STATE",
+}
+
+# name -> argument types (every function returns int status except so_last_error)
+SIGNATURES = {
+    "so_context_create": [C.c_int, _vpp],
+    "so_context_destroy": [_vp],
+    "so_context_synchronize": [_vp],
+    "so_context_stream": [_vp, _vpp],
+    "so_wave_function_create": [_vp, C.c_int, _i64p, C.c_int, _vpp],
+    "so_wave_function_set_grid": [_vp, C.c_int, C.c_char_p, C.c_int64, _dp, _dp],
+    "so_wave_function_get_grid": [_vp, C.c_int, C.c_char_p, C.c_int64, _i64p, _dp, _dp],
+    "so_wave_function_upload": [_vp, _dp],
+    "so_wave_function_download": [_vp, _dp],
+    "so_wave_function_norm": [_vp, _dp],
+    "so_wave_function_normalize": [_vp, C.c_double],
+    "so_wave_function_dot": [_vp, _vp, _dp],
+    "so_wave_function_density": [_vp, _dp],
+    "so_wave_function_state_density": [_vp, C.c_int, _dp],
+    "so_wave_function_clone": [_vp, _vpp],
+    "so_wave_function_destroy": [_vp],
+    "so_wave_function_shape": [_vp, C.POINTER(C.c_int), _i64p],
+    "so_wave_function_device_ptr": [_vp, _vpp],
+    "so_one_dim_propagator_create": [_vp, C.c_int, C.c_int64, _dp, _vpp],
+    "so_one_dim_propagator_set_operator": [_vp, C.c_int64, _dp],
+    "so_one_dim_propagator_add_operator": [_vp, C.c_int64, _dp],
+    "so_n_dim_propagator_create": [_vp, _vpp],
+    "so_n_dim_propagator_set_operator": [_vp, C.c_int, _i64p, _dp],
+    "so_n_dim_propagator_add_operator": [_vp, C.c_int, _i64p, _dp],
+    "so_non_diag_propagator_create": [_vp, C.c_int, C.c_int64, C.c_int64, _dp, _vpp],
+    "so_propagator_set_loss_checked": [_vp, C.c_char_p, C.c_int64, C.c_char_p, C.c_double, C.c_int64],
+    "so_fft_transformation_create": [_vp, C.c_int, C.c_int64, _dp, C.c_char_p, _vpp],
+    "so_matrix_transformation_create": [_vp, C.c_int, C.c_int64, C.c_char_p, _dp, _dp, _vpp],
+    "so_matrix_transformation_set_matrix": [_vp, C.c_int64, _dp, _dp],
+    "so_transformation_grid": [_vp, C.c_char_p, C.c_int64, _i64p, _dp, _dp],
+    "so_transformation_transform": [_vp, _vp],
+    "so_transformation_inverse_transform": [_vp, _vp],
+    "so_propagator_apply": [_vp, _vp],
+    "so_border_dumping_create": [_vp, C.c_int, C.c_int64, _dp, _vpp],
+    "so_leak_control_create": [_vp, _vpp],
+    "so_control_add_loss_checker": [_vp, C.c_char_p, C.c_int64, C.c_char_p, C.c_double, C.c_int64],
+    "so_control_name": [_vp, C.c_char_p, C.c_int64],
+    "so_op_loss": [_vp, _dp, C.POINTER(C.c_int)],
+    "so_op_loss_frames": [_vp, _dp, _dp, C.c_int64, _i64p],
+    "so_state_saver_create": [_vp, C.c_char_p, C.c_double, C.c_int64, C.c_int, C.c_char_p, C.c_int64, _dp,
+                              C.c_int64, _vpp],
+    "so_wave_function_saver_create": [_vp, C.c_char_p, C.c_double, C.c_int64, C.c_int64, _dp, C.c_int64, _dp,
+                                      C.c_int64, _vpp],
+    "so_saver_save": [_vp],
+    "so_saver_reset": [_vp],
+    "so_saver_frames": [_vp, _dp, _dp, _i64p],
+    "so_op_destroy": [_vp],
+    "so_operation_stack_create": [_vpp],
+    "so_operation_stack_add_propagator": [_vp, _vp],
+    "so_operation_stack_add_transformation": [_vp, _vp, C.c_int],
+    "so_operation_stack_add_saver": [_vp, _vp, C.c_int],
+    "so_operation_stack_add_control": [_vp, _vp, C.c_int],
+    "so_operation_stack_len": [_vp, _i64p],
+    "so_operation_stack_destroy": [_vp],
+    "so_propagation_create": [_vp, _vpp],
+    "so_propagation_set_wave_function": [_vp, _vp],
+    "so_propagation_set_time_grid": [_vp, C.c_double, C.c_int64, C.c_int],
+    "so_propagation_set_operation_stack": [_vp, _vp],
+    "so_propagation_wave_function": [_vp, _vpp],
+    "so_propagation_propagate": [_vp],
+    "so_propagation_step": [_vp, C.c_int64],
+    "so_propagation_mean_energy": [_vp, _dp],
+    "so_propagation_get_losses": [_vp, _dp, C.c_int64, _i64p],
+    "so_propagation_print_losses": [_vp],
+    "so_propagation_savers_save": [_vp],
+    "so_propagation_reset_savers_state": [_vp],
+    "so_propagation_reset_losses": [_vp],
+    "so_propagation_set_graphs": [_vp, C.c_int],
+    "so_propagation_stats": [_vp, _i64p, _i64p],
+    "so_propagation_destroy": [_vp],
+    "so_propagation_time_passes": [_vp, C.c_int64, C.c_int64, _vp, _i64p],
+    "so_gauss_legendre": [_vp, C.c_int64, _dp, _dp],
+}
+
+
+
+class PassTiming(C.Structure):
+    """so_pass_timing of include/split_operator_hip.h."""
+    _fields_ = [
+        ("avg_ms", C.c_double), ("points", C.c_int64), ("lane", C.c_int64), ("lanes", C.c_int64),
+        ("tiles", C.c_int64), ("mat_flops", C.c_double), ("fft_flops", C.c_double),
+        ("complex_bytes", C.c_int32), ("nx", C.c_int32), ("xform_type", C.c_int32), ("axis", C.c_int32),
+        ("lanes_per_tile", C.c_int32), ("nd_tables", C.c_int32), ("norms", C.c_int32), ("line_ops", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+class SoError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"{ERRORS.get(code, code)}: {message}")
+        self.code = code
+
+
+def load():
+    """Load the built library; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m split_operator_amd.build` "
+            "(there is no CPU fallback for the HIP path)")
+    L = C.CDLL(LIB_PATH)
+    L.so_last_error.restype = C.c_char_p
+    L.so_last_error.argtypes = []
+    for name, args in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = C.c_int
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def call(name, *args):
+    L = load()
+    rc = getattr(L, name)(*args)
+    if rc != 0:
+        raise SoError(rc, L.so_last_error().decode(errors="replace"))
+    return rc
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def i64ptr(a):
+    return a.ctypes.data_as(_i64p)

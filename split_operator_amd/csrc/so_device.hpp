@@ -1,0 +1,129 @@
+// so_device.hpp — descriptors shared by the host plan compiler and the gfx950
+// kernels of the split-operator propagation step.
+//
+// A compiled step is a list of "passes". Each pass is one kernel launch over
+// lanes of the wave function along one axis: load a tile of lanes, apply an
+// element-wise stage, run up to two lane transforms (Stockham FFT or MFMA
+// matrix) with element-wise stages between/after them, store, reduce norms in
+// FP64 and let the last workgroup run a small scalar program (loss, leak
+// control, renormalisation bookkeeping). Descriptors live in device memory and
+// are read through the scalar cache.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sod {
+
+constexpr int MAXD = 4;         // dimensions supported on the device path
+constexpr int MAX_ELEM = 12;    // element-wise ops per stage
+constexpr int MAX_ACC = 4;      // FP64 norm accumulators per pass
+constexpr int MAX_SCALAR = 32;  // scalar instructions per pass
+constexpr int MAX_RADIX = 16;   // radix passes per FFT
+constexpr int MAX_TL = 64;      // lanes per workgroup tile
+constexpr int NTHREADS = 256;   // threads per workgroup (4 wave64)
+constexpr int FFT_EPT = 32;     // complex elements per thread held in registers by an FFT radix pass
+#define SOD_MAX_DYN_LDS (64 * 1024)  // dynamic LDS bytes a pass may request per workgroup
+
+enum ElemKind : int32_t {
+  EK_MUL1D = 1,  // z *= table[i_axis]          (OneDimPropagator, BorderDumping mask)
+  EK_MULND = 2,  // z *= table[linear index]    (NDimPropagator, full table)
+  EK_SCALE = 3,  // z *= (T) S[slot]            (deferred LeakControl / normalize factor)
+  EK_NORM = 4,   // acc[k] += |z|^2 * prod_a w_a[i_a]   (FP64)
+  EK_MULB = 5,   // z *= table[0]               (NDimPropagator with a one-element operator)
+};
+
+struct ElemOp {
+  int32_t kind;
+  int32_t axis;   // EK_MUL1D: axis of the table
+  int32_t acc;    // EK_NORM: accumulator index
+  int32_t slot;   // EK_SCALE: scalar arena slot
+  const void* table;
+  const double* w_axis;  // EK_NORM: weights of the pass axis for the grid current at that point
+};
+
+struct Stage {
+  int32_t n;
+  int32_t pad;
+  ElemOp op[MAX_ELEM];
+};
+
+enum XformType : int32_t { XF_NONE = 0, XF_FFT = 1, XF_MAT = 2 };
+
+struct XformDesc {
+  int32_t type;
+  int32_t inverse;         // FFT: +1 exponent sign
+  int32_t nrad;
+  int32_t rad[MAX_RADIX];  // FFT radices, product = n
+  const void* tw;          // FFT: W[e] = exp(-2 pi i e / n), complex T, length n
+  double scale;            // FFT: 1/sqrt(n) applied after the transform
+  const void* B;           // MAT: GEMM right operand, row-major [kpad][npad] of T
+  int32_t cplx;            // MAT: 0 = real matrix (rows are (lane, re/im)), 1 = complex (real 2n x 2n form)
+  int32_t kdim;            // MAT: K (n or 2n)
+  int32_t kpad;            // MAT: K rounded up to 4
+  int32_t ncols;           // MAT: N (n or 2n)
+  int32_t npad;            // MAT: N rounded up to 16
+  int32_t pad2;
+};
+
+enum ScalarKind : int32_t {
+  SK_SET_ACC = 1,    // S[a] = acc[b]             (norm result)
+  SK_COPY = 2,       // S[a] = S[b]
+  SK_LOSS = 3,       // S[a] += S[b] - S[c]; optional history record
+  SK_SCALE = 4,      // S[a] = sqrt(S[b] / S[c])  (normalize factor)
+  SK_CONST = 5,      // S[a] = v
+};
+
+struct ScalarOp {
+  int32_t kind;
+  int32_t a, b, c;
+  double v;
+  double* hist;      // SK_LOSS: LossSaver history (device), or null
+  int32_t cnt_slot;  // SK_LOSS: slot holding the LossSaver call counter
+  int32_t pad;
+  int64_t freq;      // SK_LOSS: 2 * total / frames
+  int64_t frames;
+};
+
+struct PassDesc {
+  void* psi;         // wave function buffer (complex T, C order)
+  int32_t ndim;
+  int32_t axis;      // lane axis of the pass
+  int64_t shape[MAXD];
+  int64_t n;         // lane length = shape[axis]
+  int64_t outer;     // prod of shape before axis
+  int64_t inner;     // prod of shape after axis (lane element stride)
+  int32_t tl;        // lanes per workgroup tile
+  int32_t ld;        // LDS row pitch in complex elements
+  int64_t ntiles;
+  int64_t tiles_per_outer;  // inner > 1: tiles per outer index
+  int32_t nx;        // number of transforms (0, 1, 2)
+  int32_t nacc;      // number of norm accumulators in use
+  XformDesc xf[2];
+  Stage st[3];       // st[0] before xf[0]; st[1] after xf[0]; st[2] after xf[1]
+  const double* w_other[MAXD];  // weights of the other axes (grids fixed during the pass)
+  int32_t has_norm;
+  int32_t nscalar;
+  double* partials;  // [ntiles][MAX_ACC]
+  unsigned int* counter;
+  double* S;         // scalar arena
+  int32_t nonfinite_slot;
+  int32_t pad3;
+  ScalarOp sc[MAX_SCALAR];
+};
+
+}  // namespace sod
+
+// Launch entry points (kernels are in so_kernels.hip). `dtype` 0 = complex
+// FP32, 1 = complex FP64. Return hipError_t as int.
+extern "C++" {
+int sod_launch_pass(int dtype, const sod::PassDesc* d_desc, const sod::PassDesc& h_desc, hipStream_t s);
+int sod_launch_scalar(const sod::PassDesc* d_desc, hipStream_t s);
+int sod_launch_state_density(int dtype, const void* psi, int ndim, const int64_t* shape, int axis,
+                             const double* const* w, double* out, hipStream_t s);
+int sod_launch_density(int dtype, const void* psi, int64_t size, double* out, int64_t out_stride, hipStream_t s);
+int sod_launch_dot(int dtype, const void* a, const void* b, int ndim, const int64_t* shape,
+                   const double* const* w, double* partials, unsigned int* counter, double* out, hipStream_t s);
+int sod_launch_convert(int to_dtype, const void* src_f64, void* dst, int64_t n_complex, int direction,
+                       hipStream_t s);
+int sod_launch_gauss_legendre(int64_t n, double* x, double* w, hipStream_t s);
+}

@@ -1,0 +1,2308 @@
+// so_runtime.cpp — host runtime of the MI355X split-operator path: the C ABI
+// of include/split_operator_hip.h, host mirrors of the crate's types, and the
+// step compiler that turns one Propagation::step into fused gfx950 passes.
+//
+// How a step runs
+//   1. The operation stack is walked exactly as Propagation::step does
+//      (src/propagation.rs:134-182): forward, then reverse minus the last op,
+//      with Apply/Order gating. Walking it updates the host mirror of the
+//      crate's observable state — grid swaps (fft_transformation.rs:54,73),
+//      the ChangeObserver flags (change_observer.rs:6-47), saver frame
+//      counters — and emits abstract actions: element-wise multiplies, lane
+//      transforms, norm reductions and scalar bookkeeping.
+//   2. The compiler packs consecutive actions into passes. A pass owns one
+//      lane axis; diagonal multiplies and norms ride on the loads and stores,
+//      two transforms on the same axis (FFT -> T -> IFFT, or ILegendre ->
+//      V/2 -> Legendre across the step boundary) share one LDS tile, and the
+//      open pass is carried into the next step so the palindrome's tail fuses
+//      with the next head. Scalar work (losses, leak-control norms, the
+//      renormalisation factor) runs in the last workgroup of the pass that
+//      produced the norms; a renormalisation factor is applied as the first
+//      element-wise op of the following pass.
+//   3. Steady-state steps (same passes, same host state after each step, no
+//      saver frame) are captured once into a hipGraph of up to 64 steps and
+//      replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/split_operator_hip.h"
+#include "so_device.hpp"
+
+using sod::MAXD;
+
+// ============================================================== errors
+static thread_local std::string g_err;
+
+static int set_err(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+struct SoError {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] static void fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  throw SoError{code, buf};
+}
+
+#define HIPCHK(x)                                                                                   \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess) fail(SO_ERR_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+#define API_BEGIN try {
+#define API_END                                      \
+  }                                                  \
+  catch (const SoError& e) {                         \
+    return set_err(e.code, "%s", e.msg.c_str());     \
+  }                                                  \
+  catch (const std::exception& e) {                  \
+    return set_err(SO_ERR_INVALID, "%s", e.what());  \
+  }                                                  \
+  return SO_OK;
+
+#define NEED(p)                                                            \
+  do {                                                                     \
+    if (!(p)) fail(SO_ERR_INVALID, "null argument: %s", #p);               \
+  } while (0)
+
+extern "C" const char* so_last_error(void) { return g_err.c_str(); }
+
+// ============================================================== context
+struct so_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double* arena = nullptr;  // scalar slots (norms, losses, leak norms, factors)
+  int arena_size = 0;
+  std::vector<int> free_slots;
+  int next_slot = 0;
+  int nonfinite_slot = -1;
+  std::vector<void*> allocations;
+
+  void* alloc(size_t bytes) {
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, bytes ? bytes : 16));
+    allocations.push_back(p);
+    return p;
+  }
+  void release(void* p) {
+    if (!p) return;
+    auto it = std::find(allocations.begin(), allocations.end(), p);
+    if (it != allocations.end()) {
+      allocations.erase(it);
+      (void)hipFree(p);
+    }
+  }
+  int slot() {
+    if (!free_slots.empty()) {
+      int s = free_slots.back();
+      free_slots.pop_back();
+      return s;
+    }
+    if (next_slot >= arena_size) fail(SO_ERR_UNSUPPORTED, "scalar arena exhausted (%d slots)", arena_size);
+    return next_slot++;
+  }
+  void free_slot(int s) {
+    if (s >= 0) free_slots.push_back(s);
+  }
+  void set_slot(int s, double v) {
+    HIPCHK(hipMemcpyAsync(arena + s, &v, sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  double get_slot(int s) {
+    double v = 0;
+    HIPCHK(hipMemcpyAsync(&v, arena + s, sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return v;
+  }
+  void sync() { HIPCHK(hipStreamSynchronize(stream)); }
+};
+
+// ============================================================== grids
+struct GridH {
+  std::string name;
+  int dimension_no = 0;
+  std::vector<double> nodes, weights;
+  double* d_weights = nullptr;  // device copy (owned by the context allocation list)
+  so_context* ctx = nullptr;
+
+  const double* dev_w() {
+    if (!d_weights) {
+      d_weights = (double*)ctx->alloc(sizeof(double) * std::max<size_t>(weights.size(), 1));
+      HIPCHK(hipMemcpy(d_weights, weights.data(), sizeof(double) * weights.size(), hipMemcpyHostToDevice));
+    }
+    return d_weights;
+  }
+};
+
+// ============================================================== twiddles, radices
+// cos/sin of 2 pi x / n by octant reduction: quarter-turn values are exact.
+static void twiddle_exact(int64_t x, int64_t n, double* c, double* s) {
+  bool neg_s = false, neg_c = false;
+  if (2 * x > n) {
+    x = n - x;
+    neg_s = true;
+  }
+  if (4 * x > n) {
+    int64_t xx = n - 2 * x;
+    n = 2 * n;
+    x = xx;
+    neg_c = true;
+  }
+  double cc, ss;
+  if (8 * x > n) {
+    const double th = 2.0 * M_PI * (double)(n - 4 * x) / (double)(4 * n);
+    cc = std::sin(th);
+    ss = std::cos(th);
+  } else {
+    const double th = 2.0 * M_PI * (double)x / (double)n;
+    cc = std::cos(th);
+    ss = std::sin(th);
+  }
+  *c = neg_c ? -cc : cc;
+  *s = neg_s ? -ss : ss;
+}
+
+static bool radix_plan(int64_t n, std::vector<int>& rad) {
+  rad.clear();
+  if (n < 1) return false;
+  int64_t m = n;
+  int threes = 0;
+  while (m % 3 == 0) {
+    m /= 3;
+    threes++;
+  }
+  int twos = 0;
+  while (m % 2 == 0) {
+    m /= 2;
+    twos++;
+  }
+  if (m != 1) return false;
+  while (twos >= 3) {
+    rad.push_back(8);
+    twos -= 3;
+  }
+  if (twos == 2) rad.push_back(4);
+  if (twos == 1) rad.push_back(2);
+  for (int i = 0; i < threes; ++i) rad.push_back(3);
+  return (int)rad.size() <= sod::MAX_RADIX;
+}
+
+// ============================================================== ops
+enum OpKind { OK_ONE_DIM = 1, OK_N_DIM, OK_FFT, OK_MATRIX, OK_NON_DIAG, OK_BORDER, OK_LEAK, OK_STATE_SAVER, OK_WF_SAVER };
+
+struct LossCheckerH {
+  bool present = false;
+  std::string name;
+  int s_loss = -1, s_cur = -1, s_cnt = -1;
+  // LossSaver (loss_saver.rs:5-57)
+  bool has_saver = false;
+  std::string filename;
+  int64_t frames_no = 0, total_frames = 0;
+  double step = 0;
+  double* d_hist = nullptr;
+};
+
+// a complex table uploaded lazily in the storage dtype
+struct DevTable {
+  std::vector<double> host;  // interleaved complex f64
+  void* dev[2] = {nullptr, nullptr};
+  so_context* ctx = nullptr;
+  void invalidate() {
+    for (int i = 0; i < 2; ++i) {
+      if (dev[i]) ctx->release(dev[i]);
+      dev[i] = nullptr;
+    }
+  }
+  const void* get(int dtype) {
+    if (!dev[dtype]) {
+      const size_t n2 = host.size();
+      if (dtype == SO_COMPLEX_F64) {
+        dev[1] = ctx->alloc(sizeof(double) * std::max<size_t>(n2, 2));
+        HIPCHK(hipMemcpy(dev[1], host.data(), sizeof(double) * n2, hipMemcpyHostToDevice));
+      } else {
+        std::vector<float> f(n2);
+        for (size_t i = 0; i < n2; ++i) f[i] = (float)host[i];
+        dev[0] = ctx->alloc(sizeof(float) * std::max<size_t>(n2, 2));
+        HIPCHK(hipMemcpy(dev[0], f.data(), sizeof(float) * n2, hipMemcpyHostToDevice));
+      }
+    }
+    return dev[dtype];
+  }
+};
+
+struct so_op {
+  OpKind kind;
+  so_context* ctx = nullptr;
+  int dim = 0;
+  int64_t n = 0;
+  // diagonal operators (OneDim, NDim, Border)
+  DevTable table;
+  std::vector<int64_t> nd_shape;  // NDim operator shape
+  // transformations
+  GridH held;  // grid_transformation
+  std::vector<int> radices;
+  DevTable twiddles;
+  std::vector<double> mat_t, mat_tinv;  // complex n x n
+  void* gemm_b[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [which][dtype]
+  int gemm_cplx[2] = {0, 0};
+  // controls
+  LossCheckerH chk;
+  int s_leak = -1;  // LeakControl::norm
+  // savers
+  std::string sname;
+  double step = 0;
+  int64_t step_no = 0, frames_no = 0, current_frame = 0;
+  std::string grid_name;
+  std::vector<double> x_nodes, y_nodes;
+  int state_dim = 0;
+  double* d_frames = nullptr;  // per_frame x frames_no f64
+  int64_t per_frame = 0;
+  std::vector<double> times;
+  bool owned_by_stack = false;
+};
+
+// ============================================================== wave function
+struct Propagation;
+
+struct so_wave_function {
+  so_context* ctx = nullptr;
+  int ndim = 0;
+  int64_t shape[MAXD] = {1, 1, 1, 1};
+  int64_t size = 0;
+  int dtype = SO_COMPLEX_F32;
+  void* data = nullptr;
+  std::vector<GridH> grids;
+  bool grid_set[MAXD] = {false, false, false, false};
+  // ChangeObserver (change_observer.rs:6-12); last_norm lives in slot s_last
+  bool possible_norm_change = true;
+  int s_last = -1;
+  Propagation* owner = nullptr;  // set for a propagation's own wave function
+  double* d_partials = nullptr;  // dot scratch
+  unsigned int* d_counter = nullptr;
+  double* d_out = nullptr;
+  double* d_stage = nullptr;  // f64 staging for uploads/downloads
+};
+
+static void flush_owner(so_wave_function* wf);
+
+static size_t csize(int dtype) { return dtype == SO_COMPLEX_F64 ? 16 : 8; }
+
+static void wf_check_grids(so_wave_function* wf) {
+  for (int a = 0; a < wf->ndim; ++a)
+    if (!wf->grid_set[a]) fail(SO_ERR_STATE, "grid of axis %d not set", a);
+}
+
+static std::vector<const double*> wf_weights(so_wave_function* wf) {
+  std::vector<const double*> w(MAXD, nullptr);
+  for (int a = 0; a < wf->ndim; ++a) {
+    if ((int64_t)wf->grids[a].weights.size() != wf->shape[a])
+      fail(SO_ERR_SHAPE, "grid '%s' has %zu weights but axis %d has %lld points (shape mismatch in the crate)",
+           wf->grids[a].name.c_str(), wf->grids[a].weights.size(), a, (long long)wf->shape[a]);
+    w[a] = wf->grids[a].dev_w();
+  }
+  return w;
+}
+
+static so_wave_function* wf_new(so_context* ctx, int ndim, const int64_t* shape, int dtype) {
+  if (ndim < 1 || ndim > MAXD) fail(SO_ERR_UNSUPPORTED, "ndim %d not in 1..%d", ndim, MAXD);
+  if (dtype != SO_COMPLEX_F32 && dtype != SO_COMPLEX_F64) fail(SO_ERR_INVALID, "bad dtype %d", dtype);
+  auto* wf = new so_wave_function();
+  wf->ctx = ctx;
+  wf->ndim = ndim;
+  wf->size = 1;
+  for (int i = 0; i < ndim; ++i) {
+    if (shape[i] < 1) fail(SO_ERR_SHAPE, "axis %d has length %lld", i, (long long)shape[i]);
+    wf->shape[i] = shape[i];
+    wf->size *= shape[i];
+  }
+  wf->dtype = dtype;
+  wf->data = ctx->alloc(csize(dtype) * wf->size);
+  HIPCHK(hipMemsetAsync(wf->data, 0, csize(dtype) * wf->size, ctx->stream));
+  wf->grids.resize(ndim);
+  for (int i = 0; i < ndim; ++i) wf->grids[i].ctx = ctx;
+  wf->s_last = ctx->slot();
+  ctx->set_slot(wf->s_last, 1.0);  // ChangeObserver::new: last_norm = 1.0
+  wf->d_partials = (double*)ctx->alloc(sizeof(double) * 2 * 1024);
+  wf->d_counter = (unsigned int*)ctx->alloc(sizeof(unsigned int) * 4);
+  HIPCHK(hipMemset(wf->d_counter, 0, sizeof(unsigned int) * 4));
+  wf->d_out = (double*)ctx->alloc(sizeof(double) * 4);
+  return wf;
+}
+
+static void wf_free(so_wave_function* wf) {
+  if (!wf) return;
+  so_context* c = wf->ctx;
+  c->release(wf->data);
+  c->release(wf->d_partials);
+  c->release(wf->d_counter);
+  c->release(wf->d_out);
+  c->release(wf->d_stage);
+  for (auto& g : wf->grids) c->release(g.d_weights);
+  c->free_slot(wf->s_last);
+  delete wf;
+}
+
+static double* wf_stage(so_wave_function* wf) {
+  if (!wf->d_stage) wf->d_stage = (double*)wf->ctx->alloc(sizeof(double) * 2 * wf->size);
+  return wf->d_stage;
+}
+
+static void wf_upload(so_wave_function* wf, const double* host) {
+  so_context* c = wf->ctx;
+  if (wf->dtype == SO_COMPLEX_F64) {
+    HIPCHK(hipMemcpyAsync(wf->data, host, 16 * wf->size, hipMemcpyHostToDevice, c->stream));
+  } else {
+    double* st = wf_stage(wf);
+    HIPCHK(hipMemcpyAsync(st, host, 16 * wf->size, hipMemcpyHostToDevice, c->stream));
+    HIPCHK((hipError_t)sod_launch_convert(0, st, wf->data, wf->size, 0, c->stream));
+  }
+  c->sync();
+}
+
+static void wf_download(so_wave_function* wf, double* host) {
+  so_context* c = wf->ctx;
+  if (wf->dtype == SO_COMPLEX_F64) {
+    HIPCHK(hipMemcpyAsync(host, wf->data, 16 * wf->size, hipMemcpyDeviceToHost, c->stream));
+  } else {
+    double* st = wf_stage(wf);
+    HIPCHK((hipError_t)sod_launch_convert(0, wf->data, st, wf->size, 1, c->stream));
+    HIPCHK(hipMemcpyAsync(host, st, 16 * wf->size, hipMemcpyDeviceToHost, c->stream));
+  }
+  c->sync();
+}
+
+// ============================================================== actions and the pass compiler
+enum ActKind { A_ELEM = 1, A_NORM, A_SCALAR, A_XFORM, A_SAVE };
+
+struct XSpec {
+  int type = sod::XF_NONE;
+  int axis = 0;
+  so_op* op = nullptr;
+  int which = 0;  // FFT: 0 forward / 1 inverse; MAT: 0 transformation / 1 inverse
+};
+
+struct ElemA {
+  int kind = 0;
+  int axis = 0;
+  const void* table = nullptr;
+  int slot = -1;
+  const double* w[MAXD] = {nullptr, nullptr, nullptr, nullptr};  // NORM: weights of every axis
+};
+
+struct PassB {
+  int nx = 0;
+  XSpec xf[2];
+  std::vector<ElemA> st[3];
+  int stage = 0;
+  int nacc = 0;
+  std::vector<sod::ScalarOp> sc;
+  std::vector<int> written;  // slots written by the scalar program
+  bool empty() const { return nx == 0 && st[0].empty() && sc.empty(); }
+  bool writes(int s) const { return std::find(written.begin(), written.end(), s) != written.end(); }
+};
+
+struct CompiledPass {
+  sod::PassDesc h;
+  sod::PassDesc* d = nullptr;
+  bool scalar_only = false;
+};
+
+static void key_add(std::string& k, const void* p, size_t n) { k.append((const char*)p, n); }
+
+static std::string pass_key(const PassB& b) {
+  std::string k;
+  key_add(k, &b.nx, sizeof(int));
+  for (int i = 0; i < b.nx; ++i) {
+    key_add(k, &b.xf[i].type, sizeof(int));
+    key_add(k, &b.xf[i].axis, sizeof(int));
+    key_add(k, &b.xf[i].op, sizeof(void*));
+    key_add(k, &b.xf[i].which, sizeof(int));
+  }
+  for (int s = 0; s < 3; ++s) {
+    int n = (int)b.st[s].size();
+    key_add(k, &n, sizeof(int));
+    for (const auto& e : b.st[s]) {
+      key_add(k, &e.kind, sizeof(int));
+      key_add(k, &e.axis, sizeof(int));
+      key_add(k, &e.table, sizeof(void*));
+      key_add(k, &e.slot, sizeof(int));
+      key_add(k, e.w, sizeof(e.w));
+    }
+  }
+  int ns = (int)b.sc.size();
+  key_add(k, &ns, sizeof(int));
+  for (const auto& s : b.sc) key_add(k, &s, sizeof(s));
+  return k;
+}
+
+struct SaveAct {
+  so_op* saver = nullptr;
+  int64_t column = 0;
+};
+
+// ============================================================== propagation
+struct StackItem {
+  int type;  // 0 propagator, 1 transformation, 2 saver, 3 control
+  so_op* op;
+  int flag;
+};
+
+struct so_operation_stack {
+  std::vector<StackItem> items;
+  ~so_operation_stack() {
+    for (auto& it : items) {
+      (void)it;
+    }
+  }
+};
+
+struct Launch {
+  CompiledPass* pass = nullptr;
+  SaveAct save;
+  bool is_save = false;
+};
+
+struct Propagation {
+  so_context* ctx = nullptr;
+  so_wave_function* wf = nullptr;
+  double step = 0.0;
+  int64_t step_no = 0;
+  bool im_time = false;
+  std::vector<StackItem> stack;
+  // compiler state
+  PassB open;
+  std::vector<Launch> emitted;
+  std::unordered_map<std::string, std::unique_ptr<CompiledPass>> cache;
+  // graphs
+  bool use_graphs = true;
+  struct GraphRec {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+  };
+  std::map<std::pair<std::string, int>, GraphRec> graphs;
+  int64_t last_launches_per_step = 0;
+  std::vector<Launch> last_steady;  // launches of the last step without a saver frame
+  // owned stacks (set_operation_stack moves them in)
+  std::vector<so_operation_stack*> owned_stacks;
+
+  ~Propagation();
+  int dtype() const { return wf ? wf->dtype : SO_COMPLEX_F32; }
+
+  // ----------------------------------------------------------- action emission
+  void close_open() {
+    if (open.empty()) {
+      open = PassB();
+      return;
+    }
+    Launch L;
+    L.pass = compile(open);
+    emitted.push_back(L);
+    open = PassB();
+  }
+  void emit_elem(const ElemA& e) {
+    if (e.kind == sod::EK_SCALE && open.writes(e.slot)) close_open();
+    if ((int)open.st[open.stage].size() >= sod::MAX_ELEM) close_open();
+    open.st[open.stage].push_back(e);
+  }
+  void emit_norm() {
+    // a norm of the current state with the current grids, into slot s_last
+    if (open.nacc >= sod::MAX_ACC || (int)open.st[open.stage].size() >= sod::MAX_ELEM ||
+        (int)open.sc.size() >= sod::MAX_SCALAR)
+      close_open();
+    ElemA e;
+    e.kind = sod::EK_NORM;
+    e.slot = open.nacc;
+    auto w = wf_weights(wf);
+    for (int a = 0; a < MAXD; ++a) e.w[a] = w[a];
+    open.st[open.stage].push_back(e);
+    sod::ScalarOp s{};
+    s.kind = sod::SK_SET_ACC;
+    s.a = wf->s_last;
+    s.b = open.nacc;
+    open.sc.push_back(s);
+    open.written.push_back(wf->s_last);
+    open.nacc++;
+  }
+  void emit_scalar(const sod::ScalarOp& s) {
+    if ((int)open.sc.size() >= sod::MAX_SCALAR) close_open();
+    open.sc.push_back(s);
+    open.written.push_back(s.a);
+    if (s.kind == sod::SK_LOSS && s.hist) open.written.push_back(s.cnt_slot);
+  }
+  void emit_xform(const XSpec& x) {
+    if (open.nx == 0) {
+      open.xf[0] = x;
+      open.nx = 1;
+      open.stage = 1;
+      return;
+    }
+    if (open.nx == 1 && open.xf[0].type == x.type && open.xf[0].axis == x.axis) {
+      open.xf[1] = x;
+      open.nx = 2;
+      open.stage = 2;
+      return;
+    }
+    close_open();
+    open.xf[0] = x;
+    open.nx = 1;
+    open.stage = 1;
+  }
+  void emit_save(so_op* sv, int64_t col) {
+    close_open();
+    Launch L;
+    L.is_save = true;
+    L.save.saver = sv;
+    L.save.column = col;
+    emitted.push_back(L);
+  }
+
+  // ----------------------------------------------------------- crate semantics (host mirror)
+  // WaveFunction::norm (wave_function.rs:68-85): recompute only on a possible change
+  void sim_norm() {
+    if (!wf->possible_norm_change) return;  // cached value already in s_last
+    emit_norm();
+    wf->possible_norm_change = false;
+  }
+  void sim_copy(int dst, int src) {
+    sod::ScalarOp s{};
+    s.kind = sod::SK_COPY;
+    s.a = dst;
+    s.b = src;
+    emit_scalar(s);
+  }
+  // LossChecker::check_before / check_after (loss_checker.rs:49-63)
+  void sim_check_before(LossCheckerH& c) {
+    sim_norm();
+    sim_copy(c.s_cur, wf->s_last);
+  }
+  void sim_check_after(LossCheckerH& c) {
+    sim_norm();
+    sod::ScalarOp s{};
+    s.kind = sod::SK_LOSS;
+    s.a = c.s_loss;
+    s.b = c.s_cur;
+    s.c = wf->s_last;
+    if (c.has_saver) {
+      s.hist = c.d_hist;
+      s.cnt_slot = c.s_cnt;
+      s.freq = 2 * c.total_frames / c.frames_no;
+      s.frames = c.frames_no;
+    }
+    emit_scalar(s);
+    sim_copy(c.s_cur, wf->s_last);
+  }
+  // WaveFunction::normalize (wave_function.rs:104-109) with the norm in slot `target`
+  void sim_normalize(int target, int scale_slot) {
+    sim_norm();
+    sod::ScalarOp s{};
+    s.kind = sod::SK_SCALE;
+    s.a = scale_slot;
+    s.b = target;
+    s.c = wf->s_last;
+    emit_scalar(s);
+    ElemA e;
+    e.kind = sod::EK_SCALE;
+    e.slot = scale_slot;
+    emit_elem(e);
+    sim_copy(wf->s_last, target);
+    wf->possible_norm_change = false;
+  }
+  void sim_diag(so_op* op) {
+    ElemA e;
+    const int dt = dtype();
+    if (op->kind == OK_N_DIM) {
+      int64_t sz = 1;
+      for (auto s : op->nd_shape) sz *= s;
+      if (sz == 1) {
+        e.kind = sod::EK_MULB;
+      } else {
+        bool same = (int)op->nd_shape.size() == wf->ndim;
+        for (int a = 0; same && a < wf->ndim; ++a) same = op->nd_shape[a] == wf->shape[a];
+        if (!same) fail(SO_ERR_SHAPE, "NDimPropagator operator shape does not broadcast to the wave function");
+        e.kind = sod::EK_MULND;
+      }
+    } else {
+      if (op->dim >= wf->ndim || wf->shape[op->dim] != op->n)
+        fail(SO_ERR_SHAPE, "OneDimPropagator of length %lld does not match axis %d", (long long)op->n, op->dim);
+      e.kind = sod::EK_MUL1D;
+      e.axis = op->dim;
+    }
+    e.table = op->table.get(dt);
+    emit_elem(e);
+    wf->possible_norm_change = true;
+  }
+  // Propagator::apply (one_dim_propagator.rs:55-65, n_dim_propagator.rs:44-54)
+  void sim_apply(so_op* op) {
+    if (op->kind == OK_NON_DIAG) fail(SO_ERR_UNSUPPORTED, "NonDiagPropagator is not on the device path yet");
+    if (op->chk.present) sim_check_before(op->chk);
+    sim_diag(op);
+    if (op->chk.present) sim_check_after(op->chk);
+  }
+  // FFTTransformation / MatrixTransformation transform (fft_transformation.rs:53-88,
+  // matrix_transformation.rs:58-75): swap the grid, then transform the lanes
+  void sim_transform(so_op* op, bool inverse) {
+    if (op->dim >= wf->ndim) fail(SO_ERR_SHAPE, "transformation axis %d out of range", op->dim);
+    if (wf->shape[op->dim] != op->n)
+      fail(SO_ERR_SHAPE, "transformation of length %lld on axis %d of length %lld", (long long)op->n, op->dim,
+           (long long)wf->shape[op->dim]);
+    std::swap(wf->grids[op->dim], op->held);
+    wf->possible_norm_change = true;
+    XSpec x;
+    x.axis = op->dim;
+    x.op = op;
+    x.which = inverse ? 1 : 0;
+    if (op->kind == OK_FFT) {
+      x.type = sod::XF_FFT;
+    } else if (op->kind == OK_MATRIX) {
+      if (op->mat_t.empty()) fail(SO_ERR_STATE, "MatrixTransformation matrices not set");
+      x.type = sod::XF_MAT;
+    } else {
+      fail(SO_ERR_INVALID, "not a transformation");
+    }
+    emit_xform(x);
+  }
+  // Saver::monitor (wave_function_saver.rs:41-64, 137-151)
+  void sim_monitor(so_op* sv) {
+    if (sv->frames_no == 0) fail(SO_ERR_INVALID, "saver frames_no is zero");
+    const int64_t freq = sv->step_no / sv->frames_no;
+    if (freq == 0) fail(SO_ERR_INVALID, "saver frequency step_no / frames_no is zero (division by zero in the crate)");
+    if (sv->current_frame % freq == 0 && sv->current_frame / freq < sv->frames_no) {
+      if (sv->kind == OK_WF_SAVER && wf->ndim != 2) fail(SO_ERR_SHAPE, "Wave function must be 2d for now");
+      emit_save(sv, sv->current_frame / freq);
+      sv->times.push_back(sv->step * ((double)sv->current_frame + 1.));
+    }
+    sv->current_frame++;
+  }
+  bool monitor_is_frame(const so_op* sv, int64_t ahead) const {
+    const int64_t freq = sv->step_no / sv->frames_no;
+    const int64_t c = sv->current_frame + ahead;
+    return c % freq == 0 && c / freq < sv->frames_no;
+  }
+  // Control halves (border_dumping.rs:62-84, leak_control.rs:31-45)
+  void sim_control(so_op* c, bool second) {
+    if (c->kind == OK_LEAK) {
+      if (!second) {
+        sim_norm();
+        sim_copy(c->s_leak, wf->s_last);
+        if (c->chk.present) sim_check_before(c->chk);
+      } else {
+        if (c->chk.present) sim_check_after(c->chk);
+        sim_normalize(c->s_leak, leak_scale_slot(c));
+      }
+    } else if (c->kind == OK_BORDER) {
+      if (c->chk.present) sim_check_before(c->chk);
+      sim_diag(c);
+      if (c->chk.present) sim_check_after(c->chk);
+    } else {
+      fail(SO_ERR_INVALID, "unknown control");
+    }
+  }
+  std::map<so_op*, int> scale_slots;
+  int leak_scale_slot(so_op* c) {
+    auto it = scale_slots.find(c);
+    if (it != scale_slots.end()) return it->second;
+    int s = ctx->slot();
+    scale_slots[c] = s;
+    return s;
+  }
+
+  // Propagation::step (propagation.rs:134-182)
+  void sim_step() {
+    const int n = (int)stack.size();
+    for (int i = 0; i < n; ++i) {
+      StackItem& it = stack[i];
+      switch (it.type) {
+        case 0: sim_apply(it.op); break;
+        case 1: sim_transform(it.op, it.flag != SO_ORDER_NORMAL); break;
+        case 2:
+          if (it.flag & SO_APPLY_FIRST_HALF) sim_monitor(it.op);
+          break;
+        case 3:
+          if (it.flag & SO_APPLY_FIRST_HALF) sim_control(it.op, false);
+          break;
+      }
+    }
+    for (int i = n - 2; i >= 0; --i) {
+      StackItem& it = stack[i];
+      switch (it.type) {
+        case 0: sim_apply(it.op); break;
+        case 1: sim_transform(it.op, it.flag == SO_ORDER_NORMAL); break;
+        case 2:
+          if (it.flag & SO_APPLY_SECOND_HALF) sim_monitor(it.op);
+          break;
+        case 3:
+          if (it.flag & SO_APPLY_SECOND_HALF) sim_control(it.op, true);
+          break;
+      }
+    }
+  }
+
+  // ----------------------------------------------------------- compile
+  CompiledPass* compile(const PassB& b);
+  void launch(const Launch& L);
+  void run_saver(const SaveAct& s);
+  void flush();
+  void run_steps(int64_t n);
+  std::string state_sig() const;
+};
+
+static void flush_owner(so_wave_function* wf) {
+  if (wf && wf->owner) wf->owner->flush();
+}
+
+// GEMM operand for a matrix transformation: B = M^T (real) or the real 2n x 2n
+// form of M^T (complex), zero-padded to [kpad][npad].
+static void* gemm_operand(so_op* op, int which, int dtype, int* cplx_out) {
+  if (op->gemm_b[which][dtype]) {
+    *cplx_out = op->gemm_cplx[which];
+    return op->gemm_b[which][dtype];
+  }
+  const std::vector<double>& m = which ? op->mat_tinv : op->mat_t;
+  const int64_t n = op->n;
+  bool cplx = false;
+  for (int64_t i = 0; i < n * n; ++i)
+    if (m[2 * i + 1] != 0.0) cplx = true;
+  const int64_t K = cplx ? 2 * n : n, N = K;
+  const int64_t kpad = (K + 3) / 4 * 4, npad = (N + 15) / 16 * 16;
+  std::vector<double> B((size_t)(kpad * npad), 0.0);
+  for (int64_t r = 0; r < n; ++r)
+    for (int64_t k = 0; k < n; ++k) {
+      const double mr = m[2 * (r * n + k)], mi = m[2 * (r * n + k) + 1];
+      if (!cplx) {
+        B[k * npad + r] = mr;  // B[k][r] = M[r][k]
+      } else {
+        // y_re(r) = mr x_re(k) - mi x_im(k); y_im(r) = mi x_re(k) + mr x_im(k)
+        B[(2 * k) * npad + 2 * r] = mr;
+        B[(2 * k + 1) * npad + 2 * r] = -mi;
+        B[(2 * k) * npad + 2 * r + 1] = mi;
+        B[(2 * k + 1) * npad + 2 * r + 1] = mr;
+      }
+    }
+  void* d;
+  if (dtype == SO_COMPLEX_F64) {
+    d = op->ctx->alloc(sizeof(double) * B.size());
+    HIPCHK(hipMemcpy(d, B.data(), sizeof(double) * B.size(), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(B.size());
+    for (size_t i = 0; i < B.size(); ++i) f[i] = (float)B[i];
+    d = op->ctx->alloc(sizeof(float) * f.size());
+    HIPCHK(hipMemcpy(d, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+  }
+  op->gemm_b[which][dtype] = d;
+  op->gemm_cplx[which] = cplx ? 1 : 0;
+  *cplx_out = cplx ? 1 : 0;
+  return d;
+}
+
+static const void* fft_twiddles(so_op* op, int dtype) {
+  if (op->twiddles.host.empty()) {
+    const int64_t n = op->n;
+    op->twiddles.ctx = op->ctx;
+    op->twiddles.host.resize((size_t)(2 * n));
+    for (int64_t e = 0; e < n; ++e) {
+      double c, s;
+      twiddle_exact(e, n, &c, &s);
+      op->twiddles.host[2 * e] = c;
+      op->twiddles.host[2 * e + 1] = -s;  // exp(-2 pi i e / n)
+    }
+  }
+  return op->twiddles.get(dtype);
+}
+
+CompiledPass* Propagation::compile(const PassB& b) {
+  std::string key = pass_key(b);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second.get();
+  auto cp = std::make_unique<CompiledPass>();
+  sod::PassDesc& h = cp->h;
+  memset(&h, 0, sizeof(h));
+  const int dt = dtype();
+  h.psi = wf->data;
+  h.ndim = wf->ndim;
+  for (int a = 0; a < MAXD; ++a) h.shape[a] = a < wf->ndim ? wf->shape[a] : 1;
+  h.S = ctx->arena;
+  h.nonfinite_slot = ctx->nonfinite_slot;
+  if (b.nx == 0 && b.st[0].empty()) {
+    // scalar bookkeeping only
+    cp->scalar_only = true;
+    h.nscalar = (int)b.sc.size();
+    for (size_t i = 0; i < b.sc.size(); ++i) h.sc[i] = b.sc[i];
+    h.ntiles = 1;
+  } else {
+    const int axis = b.nx ? b.xf[0].axis : wf->ndim - 1;
+    h.axis = axis;
+    h.n = wf->shape[axis];
+    h.outer = 1;
+    h.inner = 1;
+    for (int a = 0; a < axis; ++a) h.outer *= wf->shape[a];
+    for (int a = axis + 1; a < wf->ndim; ++a) h.inner *= wf->shape[a];
+    h.nx = b.nx;
+    const int64_t n = h.n;
+    const size_t cs = csize(dt);
+    bool has_mat = false;
+    for (int i = 0; i < b.nx; ++i) {
+      const XSpec& x = b.xf[i];
+      sod::XformDesc& d = h.xf[i];
+      d.type = x.type;
+      if (x.type == sod::XF_FFT) {
+        std::vector<int> rad;
+        if (!radix_plan(n, rad))
+          fail(SO_ERR_UNSUPPORTED, "FFT length %lld is not of the form 2^a 3^b", (long long)n);
+        d.nrad = (int)rad.size();
+        for (int r = 0; r < d.nrad; ++r) d.rad[r] = rad[r];
+        d.inverse = x.which;
+        d.tw = fft_twiddles(x.op, dt);
+        d.scale = 1.0 / std::sqrt((double)n);
+      } else {
+        has_mat = true;
+        int cplx = 0;
+        d.B = gemm_operand(x.op, x.which, dt, &cplx);
+        d.cplx = cplx;
+        d.kdim = (int)(cplx ? 2 * n : n);
+        d.kpad = (d.kdim + 3) / 4 * 4;
+        d.ncols = d.kdim;
+        d.npad = (d.ncols + 15) / 16 * 16;
+      }
+    }
+    // tile choice
+    const int64_t lanes_avail = h.inner > 1 ? h.inner : h.outer;
+    int tl;
+    if (b.nx == 0) {
+      tl = (int)std::max<int64_t>(1, std::min<int64_t>(sod::MAX_TL, 4096 / std::max<int64_t>(n, 1)));
+      tl = (int)std::min<int64_t>(tl, lanes_avail);
+      h.ld = (int)n;
+    } else {
+      const int bufs = has_mat ? 2 : 1;
+      int64_t cap = sod::MAX_TL;
+      cap = std::min<int64_t>(cap, lanes_avail);
+      // LDS and FFT register budgets
+      while (cap > 1 && (int64_t)bufs * cap * (n + 1) * (int64_t)cs > SOD_MAX_DYN_LDS) cap /= 2;
+      bool has_fft = false;
+      for (int i = 0; i < b.nx; ++i) has_fft |= b.xf[i].type == sod::XF_FFT;
+      if (has_fft)
+        while (cap > 1 && cap * n > (int64_t)sod::FFT_EPT * sod::NTHREADS) cap /= 2;
+      tl = (int)std::max<int64_t>(cap, 1);
+      // prefer enough workgroups to fill 256 CUs
+      auto tiles_for = [&](int t) {
+        return h.inner > 1 ? h.outer * ((h.inner + t - 1) / t) : (h.outer + t - 1) / t;
+      };
+      const int min_tl = has_mat ? 8 : 1;
+      while (tl > min_tl && tiles_for(tl) < 512) tl /= 2;
+      if (has_fft && tl * n > (int64_t)sod::FFT_EPT * sod::NTHREADS)
+        fail(SO_ERR_UNSUPPORTED, "FFT lane length %lld exceeds the LDS tile limit", (long long)n);
+      h.ld = (int)(tl > 1 ? n + 1 : n);
+      if ((int64_t)bufs * tl * h.ld * (int64_t)cs > SOD_MAX_DYN_LDS)
+        fail(SO_ERR_UNSUPPORTED, "lane length %lld exceeds the LDS tile limit for this dtype", (long long)n);
+    }
+    h.tl = tl;
+    if (h.inner > 1) {
+      h.tiles_per_outer = (h.inner + tl - 1) / tl;
+      h.ntiles = h.outer * h.tiles_per_outer;
+    } else {
+      h.tiles_per_outer = 1;
+      h.ntiles = (h.outer + tl - 1) / tl;
+    }
+    // stages
+    bool have_other = false;
+    for (int s = 0; s < 3; ++s) {
+      h.st[s].n = (int)b.st[s].size();
+      for (size_t i = 0; i < b.st[s].size(); ++i) {
+        const ElemA& e = b.st[s][i];
+        sod::ElemOp& o = h.st[s].op[i];
+        o.kind = e.kind;
+        o.axis = e.axis;
+        o.table = e.table;
+        o.slot = e.slot;
+        if (e.kind == sod::EK_NORM) {
+          o.acc = e.slot;
+          o.slot = -1;
+          o.w_axis = e.w[axis];
+          if (!have_other) {
+            for (int a = 0; a < MAXD; ++a) h.w_other[a] = e.w[a];
+            have_other = true;
+          }
+        } else if (e.kind == sod::EK_SCALE) {
+          o.slot = e.slot;
+        }
+      }
+    }
+    h.nacc = b.nacc;
+    h.has_norm = b.nacc > 0 ? 1 : 0;
+    h.nscalar = (int)b.sc.size();
+    for (size_t i = 0; i < b.sc.size(); ++i) h.sc[i] = b.sc[i];
+    if (h.has_norm || h.nscalar) {
+      h.partials = (double*)ctx->alloc(sizeof(double) * sod::MAX_ACC * h.ntiles);
+      h.counter = (unsigned int*)ctx->alloc(sizeof(unsigned int) * 4);
+      HIPCHK(hipMemset(h.counter, 0, sizeof(unsigned int) * 4));
+    }
+  }
+  cp->d = (sod::PassDesc*)ctx->alloc(sizeof(sod::PassDesc));
+  HIPCHK(hipMemcpy(cp->d, &h, sizeof(sod::PassDesc), hipMemcpyHostToDevice));
+  CompiledPass* raw = cp.get();
+  cache.emplace(std::move(key), std::move(cp));
+  return raw;
+}
+
+void Propagation::run_saver(const SaveAct& s) {
+  so_op* sv = s.saver;
+  const int dt = dtype();
+  if (!sv->d_frames) {
+    sv->d_frames = (double*)ctx->alloc(sizeof(double) * sv->per_frame * sv->frames_no);
+    HIPCHK(hipMemsetAsync(sv->d_frames, 0, sizeof(double) * sv->per_frame * sv->frames_no, ctx->stream));
+  }
+  if (sv->kind == OK_WF_SAVER) {
+    HIPCHK((hipError_t)sod_launch_density(dt, wf->data, wf->size, sv->d_frames + s.column, sv->frames_no,
+                                          ctx->stream));
+  } else {
+    if (wf->ndim == 1) {
+      HIPCHK((hipError_t)sod_launch_density(dt, wf->data, wf->size, sv->d_frames + s.column, sv->frames_no,
+                                            ctx->stream));
+    } else {
+      // marginal into a column of the frames array: (N, frames) C order
+      auto w = wf_weights(wf);
+      const int ax = sv->state_dim;
+      if (ax >= wf->ndim || wf->shape[ax] != sv->per_frame)
+        fail(SO_ERR_SHAPE, "StateSaver grid length does not match axis %d", ax);
+      double* tmp = wf_stage(wf);
+      HIPCHK((hipError_t)sod_launch_state_density(dt, wf->data, wf->ndim, wf->shape, ax, w.data(), tmp, ctx->stream));
+      HIPCHK(hipMemcpy2DAsync(sv->d_frames + s.column, sizeof(double) * sv->frames_no, tmp, sizeof(double),
+                              sizeof(double), sv->per_frame, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+  }
+}
+
+void Propagation::launch(const Launch& L) {
+  if (L.is_save) {
+    run_saver(L.save);
+    return;
+  }
+  CompiledPass* cp = L.pass;
+  if (cp->scalar_only)
+    HIPCHK((hipError_t)sod_launch_scalar(cp->d, ctx->stream));
+  else
+    HIPCHK((hipError_t)sod_launch_pass(dtype(), cp->d, cp->h, ctx->stream));
+}
+
+void Propagation::flush() {
+  if (!open.empty()) {
+    close_open();
+    for (auto& L : emitted) launch(L);
+    emitted.clear();
+  }
+}
+
+std::string Propagation::state_sig() const {
+  std::string k = pass_key(open);
+  k.push_back(wf->possible_norm_change ? 1 : 0);
+  for (int a = 0; a < wf->ndim; ++a) {
+    k += wf->grids[a].name;
+    k.push_back(0);
+  }
+  return k;
+}
+
+Propagation::~Propagation() {
+  for (auto& kv : graphs) {
+    if (kv.second.ge) (void)hipGraphExecDestroy(kv.second.ge);
+    if (kv.second.g) (void)hipGraphDestroy(kv.second.g);
+  }
+  for (auto& kv : cache) {
+    ctx->release(kv.second->d);
+    ctx->release(kv.second->h.partials);
+    ctx->release(kv.second->h.counter);
+  }
+  for (auto& kv : scale_slots) ctx->free_slot(kv.second);
+}
+
+static std::string launches_key(const std::vector<Launch>& v) {
+  std::string k;
+  for (const auto& L : v) {
+    if (L.is_save) {
+      k.push_back('S');
+      continue;
+    }
+    key_add(k, &L.pass, sizeof(void*));
+  }
+  return k;
+}
+
+// Steps the stack n times. Steps whose launches and end state repeat the
+// previous step's are replayed from a captured hipGraph of up to 64 steps.
+void Propagation::run_steps(int64_t nsteps) {
+  std::string prev_sig = state_sig();
+  std::string prev_launch_key;
+  std::vector<Launch> prev_launches;
+  bool steady = false;
+  // monitor calls per step per saver
+  std::vector<std::pair<so_op*, int>> savers;
+  for (auto& it : stack)
+    if (it.type == 2) {
+      int c = 0;
+      if (it.flag & SO_APPLY_FIRST_HALF) c++;
+      if ((it.flag & SO_APPLY_SECOND_HALF) && &it != &stack.back()) c++;
+      savers.push_back({it.op, c});
+    }
+  int64_t done = 0;
+  while (done < nsteps) {
+    if (steady && use_graphs) {
+      // largest K <= 64 with no saver frame inside the next K steps
+      int64_t K = std::min<int64_t>(64, nsteps - done);
+      for (auto& sc : savers) {
+        for (int64_t a = 0; a < K * sc.second; ++a)
+          if (monitor_is_frame(sc.first, a)) {
+            K = std::min<int64_t>(K, a / std::max(sc.second, 1));
+            break;
+          }
+      }
+      if (K >= 2) {
+        auto gkey = std::make_pair(prev_launch_key, (int)K);
+        auto git = graphs.find(gkey);
+        if (git == graphs.end()) {
+          GraphRec rec;
+          HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+          for (int64_t s = 0; s < K; ++s)
+            for (auto& L : prev_launches) launch(L);
+          HIPCHK(hipStreamEndCapture(ctx->stream, &rec.g));
+          HIPCHK(hipGraphInstantiate(&rec.ge, rec.g, nullptr, nullptr, 0));
+          git = graphs.emplace(gkey, rec).first;
+        }
+        HIPCHK(hipGraphLaunch(git->second.ge, ctx->stream));
+        for (auto& sc : savers) sc.first->current_frame += K * sc.second;
+        done += K;
+        continue;
+      }
+    }
+    sim_step();
+    std::vector<Launch> launches;
+    launches.swap(emitted);
+    for (auto& L : launches) launch(L);
+    const std::string sig = state_sig();
+    const std::string lk = launches_key(launches);
+    bool has_save = false;
+    for (auto& L : launches) has_save |= L.is_save;
+    steady = !has_save && sig == prev_sig && lk == prev_launch_key && !launches.empty();
+    last_launches_per_step = (int64_t)launches.size();
+    if (!has_save) last_steady = launches;
+    prev_sig = sig;
+    prev_launch_key = lk;
+    prev_launches = launches;
+    done++;
+  }
+}
+
+// ============================================================== C ABI: context
+extern "C" int so_context_create(int device, so_context** out) {
+  API_BEGIN
+  NEED(out);
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) fail(SO_ERR_INVALID, "device %d not present (%d devices)", device, ndev);
+  HIPCHK(hipSetDevice(device));
+  auto* c = new so_context();
+  c->device = device;
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->arena_size = 4096;
+  c->arena = (double*)c->alloc(sizeof(double) * c->arena_size);
+  HIPCHK(hipMemset(c->arena, 0, sizeof(double) * c->arena_size));
+  c->nonfinite_slot = c->slot();
+  *out = c;
+  API_END
+}
+
+extern "C" int so_context_destroy(so_context* c) {
+  API_BEGIN
+  if (!c) return SO_OK;
+  (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocations) (void)hipFree(p);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  API_END
+}
+
+extern "C" int so_context_stream(so_context* c, void** stream) {
+  API_BEGIN
+  NEED(c);
+  NEED(stream);
+  *stream = (void*)c->stream;
+  API_END
+}
+
+extern "C" int so_context_synchronize(so_context* c) {
+  API_BEGIN
+  NEED(c);
+  c->sync();
+  API_END
+}
+
+// ============================================================== C ABI: wave function
+extern "C" int so_wave_function_create(so_context* ctx, int ndim, const int64_t* shape, int dtype,
+                                       so_wave_function** out) {
+  API_BEGIN
+  NEED(ctx);
+  NEED(shape);
+  NEED(out);
+  *out = wf_new(ctx, ndim, shape, dtype);
+  API_END
+}
+
+extern "C" int so_wave_function_set_grid(so_wave_function* wf, int axis, const char* name, int64_t n,
+                                         const double* nodes, const double* weights) {
+  API_BEGIN
+  NEED(wf);
+  NEED(name);
+  NEED(nodes);
+  NEED(weights);
+  if (axis < 0 || axis >= wf->ndim) fail(SO_ERR_INVALID, "axis %d out of range", axis);
+  if (n != wf->shape[axis]) fail(SO_ERR_SHAPE, "grid of %lld nodes for axis of %lld", (long long)n, (long long)wf->shape[axis]);
+  flush_owner(wf);
+  GridH& g = wf->grids[axis];
+  wf->ctx->release(g.d_weights);
+  g = GridH();
+  g.ctx = wf->ctx;
+  g.name = name;
+  g.dimension_no = axis;
+  g.nodes.assign(nodes, nodes + n);
+  g.weights.assign(weights, weights + n);
+  wf->grid_set[axis] = true;
+  wf->possible_norm_change = true;
+  API_END
+}
+
+extern "C" int so_wave_function_get_grid(so_wave_function* wf, int axis, char* name, int64_t cap, int64_t* n,
+                                         double* nodes, double* weights) {
+  API_BEGIN
+  NEED(wf);
+  if (axis < 0 || axis >= wf->ndim) fail(SO_ERR_INVALID, "axis %d out of range", axis);
+  const GridH& g = wf->grids[axis];
+  if (name && cap > 0) snprintf(name, (size_t)cap, "%s", g.name.c_str());
+  if (n) *n = (int64_t)g.nodes.size();
+  if (nodes) std::copy(g.nodes.begin(), g.nodes.end(), nodes);
+  if (weights) std::copy(g.weights.begin(), g.weights.end(), weights);
+  API_END
+}
+
+extern "C" int so_wave_function_upload(so_wave_function* wf, const double* data) {
+  API_BEGIN
+  NEED(wf);
+  NEED(data);
+  flush_owner(wf);
+  wf_upload(wf, data);
+  wf->possible_norm_change = true;
+  API_END
+}
+
+extern "C" int so_wave_function_download(so_wave_function* wf, double* data) {
+  API_BEGIN
+  NEED(wf);
+  NEED(data);
+  flush_owner(wf);
+  wf_download(wf, data);
+  API_END
+}
+
+// an immediate one-pass "propagation" over a free-standing wave function
+struct ImmediatePass {
+  Propagation p;
+  explicit ImmediatePass(so_wave_function* wf) {
+    p.ctx = wf->ctx;
+    p.wf = wf;
+    p.use_graphs = false;
+  }
+  ~ImmediatePass() {
+    (void)hipStreamSynchronize(p.ctx->stream);
+    p.wf = nullptr;
+  }
+  void run() {
+    p.close_open();
+    for (auto& L : p.emitted) p.launch(L);
+    p.emitted.clear();
+  }
+};
+
+static double wf_norm_value(so_wave_function* wf) {
+  flush_owner(wf);
+  wf_check_grids(wf);
+  if (wf->possible_norm_change) {
+    ImmediatePass ip(wf);
+    ip.p.sim_norm();
+    ip.run();
+  }
+  return wf->ctx->get_slot(wf->s_last);
+}
+
+static void check_nonfinite(so_context* c) {
+  const double f = c->get_slot(c->nonfinite_slot);
+  if (f != 0.0) {
+    c->set_slot(c->nonfinite_slot, 0.0);
+    fail(SO_ERR_NONFINITE, "a norm reduction produced a non-finite value");
+  }
+}
+
+extern "C" int so_wave_function_norm(so_wave_function* wf, double* out) {
+  API_BEGIN
+  NEED(wf);
+  NEED(out);
+  *out = wf_norm_value(wf);
+  API_END
+}
+
+extern "C" int so_wave_function_normalize(so_wave_function* wf, double new_norm) {
+  API_BEGIN
+  NEED(wf);
+  flush_owner(wf);
+  wf_check_grids(wf);
+  int target = wf->ctx->slot();
+  int scale = wf->ctx->slot();
+  wf->ctx->set_slot(target, new_norm);
+  {
+    ImmediatePass ip(wf);
+    ip.p.sim_normalize(target, scale);
+    ip.run();
+  }
+  wf->ctx->sync();
+  wf->ctx->free_slot(target);
+  wf->ctx->free_slot(scale);
+  API_END
+}
+
+extern "C" int so_wave_function_dot(so_wave_function* a, so_wave_function* b, double* out) {
+  API_BEGIN
+  NEED(a);
+  NEED(b);
+  NEED(out);
+  const double n1 = wf_norm_value(a);
+  const double n2 = wf_norm_value(b);
+  if (a->ndim != b->ndim || a->dtype != b->dtype) fail(SO_ERR_SHAPE, "dot: wave functions differ in shape or dtype");
+  for (int i = 0; i < a->ndim; ++i) {
+    if (a->shape[i] != b->shape[i]) fail(SO_ERR_SHAPE, "dot: shapes differ");
+    if (a->grids[i].weights != b->grids[i].weights) fail(SO_ERR_SHAPE, "dot: weight arrays differ (wave_function.rs:91)");
+  }
+  auto w = wf_weights(a);
+  HIPCHK((hipError_t)sod_launch_dot(a->dtype, a->data, b->data, a->ndim, a->shape, w.data(), a->d_partials,
+                                    a->d_counter, a->d_out, a->ctx->stream));
+  double r[2];
+  HIPCHK(hipMemcpyAsync(r, a->d_out, sizeof(r), hipMemcpyDeviceToHost, a->ctx->stream));
+  a->ctx->sync();
+  const double s = std::sqrt(n1 * n2);
+  out[0] = r[0] / s;
+  out[1] = r[1] / s;
+  API_END
+}
+
+extern "C" int so_wave_function_density(so_wave_function* wf, double* out) {
+  API_BEGIN
+  NEED(wf);
+  NEED(out);
+  flush_owner(wf);
+  double* st = wf_stage(wf);
+  HIPCHK((hipError_t)sod_launch_density(wf->dtype, wf->data, wf->size, st, 1, wf->ctx->stream));
+  HIPCHK(hipMemcpyAsync(out, st, sizeof(double) * wf->size, hipMemcpyDeviceToHost, wf->ctx->stream));
+  wf->ctx->sync();
+  API_END
+}
+
+extern "C" int so_wave_function_state_density(so_wave_function* wf, int axis, double* out) {
+  API_BEGIN
+  NEED(wf);
+  NEED(out);
+  if (axis < 0 || axis >= wf->ndim) fail(SO_ERR_INVALID, "axis out of range");
+  flush_owner(wf);
+  wf_check_grids(wf);
+  double* st = wf_stage(wf);
+  int64_t nout = wf->size;
+  if (wf->ndim == 1) {
+    HIPCHK((hipError_t)sod_launch_density(wf->dtype, wf->data, wf->size, st, 1, wf->ctx->stream));
+  } else {
+    auto w = wf_weights(wf);
+    HIPCHK((hipError_t)sod_launch_state_density(wf->dtype, wf->data, wf->ndim, wf->shape, axis, w.data(), st,
+                                                wf->ctx->stream));
+    nout = wf->shape[axis];
+  }
+  HIPCHK(hipMemcpyAsync(out, st, sizeof(double) * nout, hipMemcpyDeviceToHost, wf->ctx->stream));
+  wf->ctx->sync();
+  API_END
+}
+
+extern "C" int so_wave_function_clone(so_wave_function* wf, so_wave_function** out) {
+  API_BEGIN
+  NEED(wf);
+  NEED(out);
+  flush_owner(wf);
+  so_wave_function* c = wf_new(wf->ctx, wf->ndim, wf->shape, wf->dtype);
+  HIPCHK(hipMemcpyAsync(c->data, wf->data, csize(wf->dtype) * wf->size, hipMemcpyDeviceToDevice, wf->ctx->stream));
+  for (int a = 0; a < wf->ndim; ++a) {
+    c->grids[a].name = wf->grids[a].name;
+    c->grids[a].dimension_no = wf->grids[a].dimension_no;
+    c->grids[a].nodes = wf->grids[a].nodes;
+    c->grids[a].weights = wf->grids[a].weights;
+    c->grid_set[a] = wf->grid_set[a];
+  }
+  c->possible_norm_change = wf->possible_norm_change;
+  const double last = wf->ctx->get_slot(wf->s_last);
+  wf->ctx->set_slot(c->s_last, last);
+  wf->ctx->sync();
+  *out = c;
+  API_END
+}
+
+extern "C" int so_wave_function_destroy(so_wave_function* wf) {
+  API_BEGIN
+  if (!wf) return SO_OK;
+  if (wf->owner) fail(SO_ERR_INVALID, "this wave function belongs to a propagation");
+  wf->ctx->sync();
+  wf_free(wf);
+  API_END
+}
+
+extern "C" int so_wave_function_shape(so_wave_function* wf, int* ndim, int64_t* shape) {
+  API_BEGIN
+  NEED(wf);
+  NEED(ndim);
+  *ndim = wf->ndim;
+  if (shape)
+    for (int a = 0; a < wf->ndim; ++a) shape[a] = wf->shape[a];
+  API_END
+}
+
+extern "C" int so_wave_function_device_ptr(so_wave_function* wf, void** ptr) {
+  API_BEGIN
+  NEED(wf);
+  NEED(ptr);
+  flush_owner(wf);
+  *ptr = wf->data;
+  API_END
+}
+
+// ============================================================== C ABI: ops
+static so_op* op_new(so_context* ctx, OpKind k) {
+  auto* o = new so_op();
+  o->kind = k;
+  o->ctx = ctx;
+  o->table.ctx = ctx;
+  o->twiddles.ctx = ctx;
+  o->held.ctx = ctx;
+  return o;
+}
+
+extern "C" int so_one_dim_propagator_create(so_context* ctx, int dim, int64_t n, const double* op, so_op** out) {
+  API_BEGIN
+  NEED(ctx);
+  NEED(op);
+  NEED(out);
+  if (n < 1 || dim < 0) fail(SO_ERR_INVALID, "bad length or dimension");
+  so_op* o = op_new(ctx, OK_ONE_DIM);
+  o->dim = dim;
+  o->n = n;
+  o->table.host.assign(op, op + 2 * n);
+  *out = o;
+  API_END
+}
+
+extern "C" int so_one_dim_propagator_set_operator(so_op* p, int64_t n, const double* op) {
+  API_BEGIN
+  NEED(p);
+  NEED(op);
+  if (p->kind != OK_ONE_DIM && p->kind != OK_BORDER) fail(SO_ERR_INVALID, "not a OneDimPropagator");
+  if (n != p->n) fail(SO_ERR_SHAPE, "operator length %lld != %lld (one_dim_propagator.rs:26)", (long long)n, (long long)p->n);
+  p->table.invalidate();
+  p->table.host.assign(op, op + 2 * n);
+  API_END
+}
+
+extern "C" int so_one_dim_propagator_add_operator(so_op* p, int64_t n, const double* op) {
+  API_BEGIN
+  NEED(p);
+  NEED(op);
+  if (p->kind != OK_ONE_DIM) fail(SO_ERR_INVALID, "not a OneDimPropagator");
+  if (n != p->n) fail(SO_ERR_SHAPE, "operator length %lld != %lld (one_dim_propagator.rs:32)", (long long)n, (long long)p->n);
+  p->table.invalidate();
+  for (int64_t i = 0; i < n; ++i) {
+    const double a = p->table.host[2 * i], b = p->table.host[2 * i + 1];
+    p->table.host[2 * i] = a * op[2 * i] - b * op[2 * i + 1];
+    p->table.host[2 * i + 1] = a * op[2 * i + 1] + b * op[2 * i];
+  }
+  API_END
+}
+
+extern "C" int so_n_dim_propagator_create(so_context* ctx, so_op** out) {
+  API_BEGIN
+  NEED(ctx);
+  NEED(out);
+  so_op* o = op_new(ctx, OK_N_DIM);
+  o->nd_shape = {1};
+  o->table.host = {0.0, 0.0};  // zeros([1]) (n_dim_propagator.rs:17)
+  *out = o;
+  API_END
+}
+
+extern "C" int so_n_dim_propagator_set_operator(so_op* p, int ndim, const int64_t* shape, const double* op) {
+  API_BEGIN
+  NEED(p);
+  NEED(shape);
+  NEED(op);
+  if (p->kind != OK_N_DIM) fail(SO_ERR_INVALID, "not an NDimPropagator");
+  if (ndim < 1 || ndim > MAXD) fail(SO_ERR_UNSUPPORTED, "ndim %d", ndim);
+  int64_t sz = 1;
+  for (int i = 0; i < ndim; ++i) sz *= shape[i];
+  p->table.invalidate();
+  p->nd_shape.assign(shape, shape + ndim);
+  p->table.host.assign(op, op + 2 * sz);
+  API_END
+}
+
+extern "C" int so_n_dim_propagator_add_operator(so_op* p, int ndim, const int64_t* shape, const double* op) {
+  API_BEGIN
+  NEED(p);
+  NEED(shape);
+  NEED(op);
+  if (p->kind != OK_N_DIM) fail(SO_ERR_INVALID, "not an NDimPropagator");
+  if ((int)p->nd_shape.size() != ndim) fail(SO_ERR_SHAPE, "operator shape differs (n_dim_propagator.rs:27)");
+  int64_t sz = 1;
+  for (int i = 0; i < ndim; ++i) {
+    if (shape[i] != p->nd_shape[i]) fail(SO_ERR_SHAPE, "operator shape differs (n_dim_propagator.rs:27)");
+    sz *= shape[i];
+  }
+  p->table.invalidate();
+  for (int64_t i = 0; i < sz; ++i) {
+    const double a = p->table.host[2 * i], b = p->table.host[2 * i + 1];
+    p->table.host[2 * i] = a * op[2 * i] - b * op[2 * i + 1];
+    p->table.host[2 * i + 1] = a * op[2 * i + 1] + b * op[2 * i];
+  }
+  API_END
+}
+
+extern "C" int so_non_diag_propagator_create(so_context* ctx, int dim, int64_t lanes, int64_t n, const double* ops,
+                                             so_op** out) {
+  API_BEGIN
+  NEED(ctx);
+  NEED(ops);
+  NEED(out);
+  so_op* o = op_new(ctx, OK_NON_DIAG);
+  o->dim = dim;
+  o->n = n;
+  o->mat_t.assign(ops, ops + 2 * lanes * n * n);
+  *out = o;
+  API_END
+}
+
+static void set_checker(so_op* p, const char* name, int64_t frames_no, const char* filename, double step,
+                        int64_t step_no) {
+  LossCheckerH& c = p->chk;
+  so_context* ctx = p->ctx;
+  if (!c.present) {
+    c.s_loss = ctx->slot();
+    c.s_cur = ctx->slot();
+    c.s_cnt = ctx->slot();
+  }
+  c.present = true;
+  c.name = name ? name : "";
+  ctx->set_slot(c.s_loss, 0.0);
+  ctx->set_slot(c.s_cur, 1.0);
+  ctx->set_slot(c.s_cnt, 0.0);
+  c.has_saver = frames_no > 0;
+  if (c.has_saver) {
+    if (2 * step_no / frames_no == 0) fail(SO_ERR_INVALID, "LossSaver frequency 2*step_no/frames_no is zero");
+    c.filename = filename ? filename : c.name;
+    c.frames_no = frames_no;
+    c.total_frames = step_no;
+    c.step = step;
+    ctx->release(c.d_hist);
+    c.d_hist = (double*)ctx->alloc(sizeof(double) * frames_no);
+    HIPCHK(hipMemset(c.d_hist, 0, sizeof(double) * frames_no));
+  }
+  ctx->sync();
+}
+
+extern "C" int so_propagator_set_loss_checked(so_op* p, const char* name, int64_t frames_no, const char* filename,
+                                              double step, int64_t step_no) {
+  API_BEGIN
+  NEED(p);
+  if (p->kind != OK_ONE_DIM && p->kind != OK_N_DIM && p->kind != OK_NON_DIAG) fail(SO_ERR_INVALID, "not a propagator");
+  set_checker(p, name, frames_no, filename, step, step_no);
+  API_END
+}
+
+extern "C" int so_control_add_loss_checker(so_op* p, const char* name, int64_t frames_no, const char* filename,
+                                           double step, int64_t step_no) {
+  API_BEGIN
+  NEED(p);
+  if (p->kind != OK_BORDER && p->kind != OK_LEAK) fail(SO_ERR_INVALID, "not a control");
+  set_checker(p, name, frames_no, filename, step, step_no);
+  API_END
+}
+
+extern "C" int so_control_name(so_op* c, char* name, int64_t cap) {
+  API_BEGIN
+  NEED(c);
+  NEED(name);
+  const char* s = c->kind == OK_LEAK ? "LeakControl" : c->kind == OK_BORDER ? "BorderDumping" : "";
+  snprintf(name, (size_t)cap, "%s", s);
+  API_END
+}
+
+// LossSaver::save (loss_saver.rs:41-56): "time\tlosses for <name>" then rows
+static int write_loss_file(so_op* op) {
+  LossCheckerH& c = op->chk;
+  const int64_t freq = 2 * c.total_frames / c.frames_no;
+  const int64_t calls = (int64_t)op->ctx->get_slot(c.s_cnt);
+  int64_t nrec = 0;
+  for (int64_t f = 0; f < calls; ++f)
+    if (f % freq == 0 && f / freq < c.frames_no) nrec++;
+  std::vector<double> hist((size_t)c.frames_no);
+  HIPCHK(hipMemcpy(hist.data(), c.d_hist, sizeof(double) * c.frames_no, hipMemcpyDeviceToHost));
+  std::string path = c.filename + ".dat";
+  FILE* f = fopen(path.c_str(), "w");
+  if (!f) return set_err(SO_ERR_IO, "cannot write %s", path.c_str());
+  fprintf(f, "time\tlosses for %s\n", c.filename.c_str());
+  int64_t r = 0;
+  for (int64_t call = 0; call < calls && r < nrec; ++call)
+    if (call % freq == 0) {
+      fprintf(f, "%.17g\t%.17g\n", ((double)call + 1.0) / 2. * c.step, hist[r]);
+      r++;
+    }
+  fclose(f);
+  return SO_OK;
+}
+
+extern "C" int so_op_loss(so_op* op, double* out, int* present) {
+  API_BEGIN
+  NEED(op);
+  NEED(out);
+  if (present) *present = op->chk.present ? 1 : 0;
+  if (!op->chk.present) {
+    *out = 0.0;
+    return SO_OK;
+  }
+  *out = op->ctx->get_slot(op->chk.s_loss);
+  if (op->chk.has_saver) {
+    int rc = write_loss_file(op);
+    if (rc) return rc;
+  }
+  API_END
+}
+
+extern "C" int so_op_loss_frames(so_op* op, double* times, double* losses, int64_t cap, int64_t* n) {
+  API_BEGIN
+  NEED(op);
+  NEED(n);
+  *n = 0;
+  LossCheckerH& c = op->chk;
+  if (!c.present || !c.has_saver) return SO_OK;
+  const int64_t freq = 2 * c.total_frames / c.frames_no;
+  const int64_t calls = (int64_t)op->ctx->get_slot(c.s_cnt);
+  std::vector<double> hist((size_t)c.frames_no);
+  HIPCHK(hipMemcpy(hist.data(), c.d_hist, sizeof(double) * c.frames_no, hipMemcpyDeviceToHost));
+  int64_t r = 0;
+  for (int64_t call = 0; call < calls; ++call)
+    if (call % freq == 0 && call / freq < c.frames_no) {
+      if (r < cap) {
+        if (times) times[r] = ((double)call + 1.0) / 2. * c.step;
+        if (losses) losses[r] = hist[r];
+      }
+      r++;
+    }
+  *n = r;
+  API_END
+}
+
+extern "C" int so_fft_transformation_create(so_context* ctx, int dim, int64_t n, const double* nodes,
+                                            const char* tname, so_op** out) {
+  API_BEGIN
+  NEED(ctx);
+  NEED(nodes);
+  NEED(tname);
+  NEED(out);
+  if (n < 2) fail(SO_ERR_INVALID, "FFT grid needs at least 2 nodes");
+  if (n % 2 != 0)
+    fail(SO_ERR_UNSUPPORTED, "odd FFT length %lld: the crate's momentum grid has N-1 entries (kinetic_operator.rs:12-15)",
+         (long long)n);
+  std::vector<int> rad;
+  if (!radix_plan(n, rad)) fail(SO_ERR_UNSUPPORTED, "FFT length %lld is not of the form 2^a 3^b", (long long)n);
+  so_op* o = op_new(ctx, OK_FFT);
+  o->dim = dim;
+  o->n = n;
+  o->radices = rad;
+  // momentum grid (fft_transformation.rs:28-39)
+  const double dk = 2.0 * M_PI / (nodes[n - 1] - nodes[0]) * (1. - 1. / (double)n);
+  o->held.name = tname;
+  o->held.dimension_no = dim;
+  for (int64_t x = 0; x < n / 2; ++x) o->held.nodes.push_back((double)x * dk);
+  for (int64_t x = -n / 2; x < 0; ++x) o->held.nodes.push_back((double)x * dk);
+  o->held.weights.assign((size_t)n, dk);
+  o->held.weights[n / 2 - 1] *= 0.5;
+  o->held.weights[n / 2] *= 0.5;
+  *out = o;
+  API_END
+}
+
+extern "C" int so_matrix_transformation_create(so_context* ctx, int dim, int64_t n, const char* gname,
+                                               const double* gnodes, const double* gweights, so_op** out) {
+  API_BEGIN
+  NEED(ctx);
+  NEED(gname);
+  NEED(gnodes);
+  NEED(gweights);
+  NEED(out);
+  if (n < 1) fail(SO_ERR_INVALID, "bad length");
+  so_op* o = op_new(ctx, OK_MATRIX);
+  o->dim = dim;
+  o->n = n;
+  o->held.name = gname;
+  o->held.dimension_no = dim;
+  o->held.nodes.assign(gnodes, gnodes + n);
+  o->held.weights.assign(gweights, gweights + n);
+  *out = o;
+  API_END
+}
+
+extern "C" int so_matrix_transformation_set_matrix(so_op* t, int64_t n, const double* m, const double* minv) {
+  API_BEGIN
+  NEED(t);
+  NEED(m);
+  NEED(minv);
+  if (t->kind != OK_MATRIX) fail(SO_ERR_INVALID, "not a MatrixTransformation");
+  if (n != t->n) fail(SO_ERR_SHAPE, "matrix of size %lld for a lane of %lld (matrix_transformation.rs:40-47)",
+                      (long long)n, (long long)t->n);
+  t->mat_t.assign(m, m + 2 * n * n);
+  t->mat_tinv.assign(minv, minv + 2 * n * n);
+  for (int w = 0; w < 2; ++w)
+    for (int d = 0; d < 2; ++d) {
+      t->ctx->release(t->gemm_b[w][d]);
+      t->gemm_b[w][d] = nullptr;
+    }
+  API_END
+}
+
+extern "C" int so_transformation_grid(so_op* t, char* name, int64_t cap, int64_t* n, double* nodes, double* weights) {
+  API_BEGIN
+  NEED(t);
+  if (t->kind != OK_FFT && t->kind != OK_MATRIX) fail(SO_ERR_INVALID, "not a transformation");
+  const GridH& g = t->held;
+  if (name && cap > 0) snprintf(name, (size_t)cap, "%s", g.name.c_str());
+  if (n) *n = (int64_t)g.nodes.size();
+  if (nodes) std::copy(g.nodes.begin(), g.nodes.end(), nodes);
+  if (weights) std::copy(g.weights.begin(), g.weights.end(), weights);
+  API_END
+}
+
+static void direct_apply(so_op* op, so_wave_function* wf, int mode) {
+  flush_owner(wf);
+  wf_check_grids(wf);
+  if (wf->owner) fail(SO_ERR_INVALID, "apply ops to a free-standing wave function, not a propagation's");
+  ImmediatePass ip(wf);
+  if (mode == 0)
+    ip.p.sim_apply(op);
+  else
+    ip.p.sim_transform(op, mode == 2);
+  ip.run();
+  wf->ctx->sync();
+}
+
+extern "C" int so_transformation_transform(so_op* t, so_wave_function* wf) {
+  API_BEGIN
+  NEED(t);
+  NEED(wf);
+  direct_apply(t, wf, 1);
+  API_END
+}
+extern "C" int so_transformation_inverse_transform(so_op* t, so_wave_function* wf) {
+  API_BEGIN
+  NEED(t);
+  NEED(wf);
+  direct_apply(t, wf, 2);
+  API_END
+}
+extern "C" int so_propagator_apply(so_op* p, so_wave_function* wf) {
+  API_BEGIN
+  NEED(p);
+  NEED(wf);
+  if (p->kind != OK_ONE_DIM && p->kind != OK_N_DIM && p->kind != OK_NON_DIAG) fail(SO_ERR_INVALID, "not a propagator");
+  direct_apply(p, wf, 0);
+  API_END
+}
+
+extern "C" int so_border_dumping_create(so_context* ctx, int dim, int64_t n, const double* mask, so_op** out) {
+  API_BEGIN
+  NEED(ctx);
+  NEED(mask);
+  NEED(out);
+  so_op* o = op_new(ctx, OK_BORDER);
+  o->dim = dim;
+  o->n = n;
+  o->table.host.assign(mask, mask + 2 * n);
+  *out = o;
+  API_END
+}
+
+extern "C" int so_leak_control_create(so_context* ctx, so_op** out) {
+  API_BEGIN
+  NEED(ctx);
+  NEED(out);
+  so_op* o = op_new(ctx, OK_LEAK);
+  o->s_leak = ctx->slot();
+  ctx->set_slot(o->s_leak, 0.0);
+  *out = o;
+  API_END
+}
+
+extern "C" int so_state_saver_create(so_context* ctx, const char* name, double step, int64_t step_no, int dim,
+                                     const char* gname, int64_t n, const double* nodes, int64_t frames_no,
+                                     so_op** out) {
+  API_BEGIN
+  NEED(ctx);
+  NEED(name);
+  NEED(gname);
+  NEED(nodes);
+  NEED(out);
+  so_op* o = op_new(ctx, OK_STATE_SAVER);
+  o->sname = name;
+  o->step = step;
+  o->step_no = step_no;
+  o->state_dim = dim;
+  o->grid_name = gname;
+  o->x_nodes.assign(nodes, nodes + n);
+  o->frames_no = frames_no;
+  o->per_frame = n;
+  *out = o;
+  API_END
+}
+
+extern "C" int so_wave_function_saver_create(so_context* ctx, const char* name, double step, int64_t step_no,
+                                             int64_t nx, const double* xn, int64_t ny, const double* yn,
+                                             int64_t frames_no, so_op** out) {
+  API_BEGIN
+  NEED(ctx);
+  NEED(name);
+  NEED(xn);
+  NEED(yn);
+  NEED(out);
+  so_op* o = op_new(ctx, OK_WF_SAVER);
+  o->sname = name;
+  o->step = step;
+  o->step_no = step_no;
+  o->x_nodes.assign(xn, xn + nx);
+  o->y_nodes.assign(yn, yn + ny);
+  o->frames_no = frames_no;
+  o->per_frame = nx * ny;
+  *out = o;
+  API_END
+}
+
+// minimal .npy writer: little-endian f64, C order (the layout write_npy emits)
+static bool write_npy(const std::string& path, const double* data, const std::vector<int64_t>& shape) {
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) return false;
+  std::string sh = "(";
+  for (size_t i = 0; i < shape.size(); ++i) {
+    sh += std::to_string(shape[i]);
+    sh += (shape.size() == 1 || i + 1 < shape.size()) ? "," : "";
+    if (i + 1 < shape.size()) sh += " ";
+  }
+  sh += ")";
+  std::string hdr = "{'descr': '<f8', 'fortran_order': False, 'shape': " + sh + ", }";
+  size_t total = 10 + hdr.size() + 1;
+  size_t padn = (64 - total % 64) % 64;
+  hdr.append(padn, ' ');
+  hdr.push_back('\n');
+  const unsigned short hl = (unsigned short)hdr.size();
+  const unsigned char magic[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+  fwrite(magic, 1, 8, f);
+  fwrite(&hl, 2, 1, f);
+  fwrite(hdr.data(), 1, hdr.size(), f);
+  int64_t cnt = 1;
+  for (auto s : shape) cnt *= s;
+  fwrite(data, sizeof(double), (size_t)cnt, f);
+  fclose(f);
+  return true;
+}
+
+static std::vector<double> saver_host_frames(so_op* s) {
+  std::vector<double> h((size_t)(s->per_frame * s->frames_no), 0.0);
+  if (s->d_frames) {
+    s->ctx->sync();
+    HIPCHK(hipMemcpy(h.data(), s->d_frames, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+  }
+  return h;
+}
+
+extern "C" int so_saver_save(so_op* s) {
+  API_BEGIN
+  NEED(s);
+  if (s->kind != OK_STATE_SAVER && s->kind != OK_WF_SAVER) fail(SO_ERR_INVALID, "not a saver");
+  auto h = saver_host_frames(s);
+  if (s->kind == OK_WF_SAVER) {
+    if (!write_npy(s->sname + ".npy", h.data(), {(int64_t)s->x_nodes.size(), (int64_t)s->y_nodes.size(), s->frames_no}))
+      fail(SO_ERR_IO, "Failed to save wave function");
+    if (!write_npy(s->sname + "_x_grid.npy", s->x_nodes.data(), {(int64_t)s->x_nodes.size()}))
+      fail(SO_ERR_IO, "Failed to save r grid");
+    if (!write_npy(s->sname + "_y_grid.npy", s->y_nodes.data(), {(int64_t)s->y_nodes.size()}))
+      fail(SO_ERR_IO, "Failed to save theta grid");
+  } else {
+    if (!write_npy(s->sname + ".npy", h.data(), {s->per_frame, s->frames_no}))
+      fail(SO_ERR_IO, "Failed to save wave function");
+    if (!write_npy(s->sname + "_" + s->grid_name + "_grid.npy", s->x_nodes.data(), {(int64_t)s->x_nodes.size()}))
+      fail(SO_ERR_IO, "Failed to save state grid");
+  }
+  if (!write_npy(s->sname + "_time.npy", s->times.data(), {(int64_t)s->times.size()}))
+    fail(SO_ERR_IO, "Failed to save time grid");
+  API_END
+}
+
+extern "C" int so_saver_reset(so_op* s) {
+  API_BEGIN
+  NEED(s);
+  if (s->d_frames) HIPCHK(hipMemsetAsync(s->d_frames, 0, sizeof(double) * s->per_frame * s->frames_no, s->ctx->stream));
+  API_END
+}
+
+extern "C" int so_saver_frames(so_op* s, double* data, double* times, int64_t* n_times) {
+  API_BEGIN
+  NEED(s);
+  if (s->kind != OK_STATE_SAVER && s->kind != OK_WF_SAVER) fail(SO_ERR_INVALID, "not a saver");
+  auto h = saver_host_frames(s);
+  if (data) std::copy(h.begin(), h.end(), data);
+  if (times) std::copy(s->times.begin(), s->times.end(), times);
+  if (n_times) *n_times = (int64_t)s->times.size();
+  API_END
+}
+
+static void op_free(so_op* o) {
+  if (!o) return;
+  so_context* c = o->ctx;
+  o->table.invalidate();
+  o->twiddles.invalidate();
+  for (int w = 0; w < 2; ++w)
+    for (int d = 0; d < 2; ++d) c->release(o->gemm_b[w][d]);
+  c->release(o->held.d_weights);
+  c->release(o->d_frames);
+  c->release(o->chk.d_hist);
+  if (o->chk.present) {
+    c->free_slot(o->chk.s_loss);
+    c->free_slot(o->chk.s_cur);
+    c->free_slot(o->chk.s_cnt);
+  }
+  c->free_slot(o->s_leak);
+  delete o;
+}
+
+extern "C" int so_op_destroy(so_op* op) {
+  API_BEGIN
+  if (!op) return SO_OK;
+  if (op->owned_by_stack) fail(SO_ERR_INVALID, "op is owned by an operation stack");
+  op->ctx->sync();
+  op_free(op);
+  API_END
+}
+
+// ============================================================== C ABI: stack
+extern "C" int so_operation_stack_create(so_operation_stack** out) {
+  API_BEGIN
+  NEED(out);
+  *out = new so_operation_stack();
+  API_END
+}
+
+static void stack_push(so_operation_stack* s, so_op* op, int type, int flag) {
+  NEED(s);
+  NEED(op);
+  if (op->owned_by_stack) fail(SO_ERR_INVALID, "op already added to a stack");
+  op->owned_by_stack = true;
+  s->items.push_back({type, op, flag});
+}
+
+extern "C" int so_operation_stack_add_propagator(so_operation_stack* s, so_op* p) {
+  API_BEGIN
+  NEED(p);
+  if (p->kind != OK_ONE_DIM && p->kind != OK_N_DIM && p->kind != OK_NON_DIAG) fail(SO_ERR_INVALID, "not a propagator");
+  stack_push(s, p, 0, 0);
+  API_END
+}
+extern "C" int so_operation_stack_add_transformation(so_operation_stack* s, so_op* t, int order) {
+  API_BEGIN
+  NEED(t);
+  if (t->kind != OK_FFT && t->kind != OK_MATRIX) fail(SO_ERR_INVALID, "not a transformation");
+  if (order != SO_ORDER_NORMAL && order != SO_ORDER_INVERSE_FIRST) fail(SO_ERR_INVALID, "bad order");
+  stack_push(s, t, 1, order);
+  API_END
+}
+extern "C" int so_operation_stack_add_saver(so_operation_stack* s, so_op* sv, int apply) {
+  API_BEGIN
+  NEED(sv);
+  if (sv->kind != OK_STATE_SAVER && sv->kind != OK_WF_SAVER) fail(SO_ERR_INVALID, "not a saver");
+  if (apply == SO_APPLY_NONE) fail(SO_ERR_INVALID, "saver apply must not be Apply::None (propagation.rs:56)");
+  stack_push(s, sv, 2, apply);
+  API_END
+}
+extern "C" int so_operation_stack_add_control(so_operation_stack* s, so_op* c, int apply) {
+  API_BEGIN
+  NEED(c);
+  if (c->kind != OK_BORDER && c->kind != OK_LEAK) fail(SO_ERR_INVALID, "not a control");
+  stack_push(s, c, 3, apply);
+  API_END
+}
+extern "C" int so_operation_stack_len(so_operation_stack* s, int64_t* out) {
+  API_BEGIN
+  NEED(s);
+  NEED(out);
+  *out = (int64_t)s->items.size();
+  API_END
+}
+extern "C" int so_operation_stack_destroy(so_operation_stack* s) {
+  API_BEGIN
+  if (!s) return SO_OK;
+  for (auto& it : s->items) {
+    it.op->owned_by_stack = false;
+    op_free(it.op);
+  }
+  delete s;
+  API_END
+}
+
+// ============================================================== C ABI: propagation
+struct so_propagation {
+  Propagation p;
+  so_operation_stack* stack = nullptr;
+};
+
+extern "C" int so_propagation_create(so_context* ctx, so_propagation** out) {
+  API_BEGIN
+  NEED(ctx);
+  NEED(out);
+  auto* pr = new so_propagation();
+  pr->p.ctx = ctx;
+  *out = pr;
+  API_END
+}
+
+static void prop_reset_compiled(Propagation& p) {
+  if (p.wf) p.flush();
+  p.ctx->sync();
+  for (auto& kv : p.graphs) {
+    if (kv.second.ge) (void)hipGraphExecDestroy(kv.second.ge);
+    if (kv.second.g) (void)hipGraphDestroy(kv.second.g);
+  }
+  p.graphs.clear();
+  for (auto& kv : p.cache) {
+    p.ctx->release(kv.second->d);
+    p.ctx->release(kv.second->h.partials);
+    p.ctx->release(kv.second->h.counter);
+  }
+  p.cache.clear();
+  p.open = PassB();
+  p.emitted.clear();
+}
+
+extern "C" int so_propagation_set_wave_function(so_propagation* pr, so_wave_function* src) {
+  API_BEGIN
+  NEED(pr);
+  NEED(src);
+  flush_owner(src);
+  wf_check_grids(src);
+  Propagation& p = pr->p;
+  prop_reset_compiled(p);
+  if (p.wf) {
+    p.wf->owner = nullptr;
+    wf_free(p.wf);
+    p.wf = nullptr;
+  }
+  so_wave_function* c = nullptr;
+  if (so_wave_function_clone(src, &c) != SO_OK) fail(SO_ERR_HIP, "%s", g_err.c_str());
+  c->owner = &p;
+  p.wf = c;
+  API_END
+}
+
+extern "C" int so_propagation_set_time_grid(so_propagation* pr, double step, int64_t step_no, int im_time) {
+  API_BEGIN
+  NEED(pr);
+  pr->p.step = step;
+  pr->p.step_no = step_no;
+  pr->p.im_time = im_time != 0;
+  API_END
+}
+
+extern "C" int so_propagation_set_operation_stack(so_propagation* pr, so_operation_stack* s) {
+  API_BEGIN
+  NEED(pr);
+  NEED(s);
+  Propagation& p = pr->p;
+  prop_reset_compiled(p);
+  if (pr->stack) so_operation_stack_destroy(pr->stack);
+  pr->stack = s;
+  p.stack = s->items;
+  API_END
+}
+
+extern "C" int so_propagation_wave_function(so_propagation* pr, so_wave_function** out) {
+  API_BEGIN
+  NEED(pr);
+  NEED(out);
+  if (!pr->p.wf) fail(SO_ERR_STATE, "no wave function set");
+  *out = pr->p.wf;
+  API_END
+}
+
+static void prop_ready(Propagation& p) {
+  if (!p.wf) fail(SO_ERR_STATE, "no wave function set");
+  if (p.stack.empty()) fail(SO_ERR_STATE, "empty operation stack");
+}
+
+extern "C" int so_propagation_step(so_propagation* pr, int64_t n) {
+  API_BEGIN
+  NEED(pr);
+  prop_ready(pr->p);
+  pr->p.run_steps(n);
+  API_END
+}
+
+extern "C" int so_propagation_propagate(so_propagation* pr) {
+  API_BEGIN
+  NEED(pr);
+  prop_ready(pr->p);
+  pr->p.run_steps(pr->p.step_no);
+  pr->p.flush();
+  pr->p.ctx->sync();
+  check_nonfinite(pr->p.ctx);
+  API_END
+}
+
+extern "C" int so_propagation_mean_energy(so_propagation* pr, double* out) {
+  API_BEGIN
+  NEED(pr);
+  NEED(out);
+  Propagation& p = pr->p;
+  prop_ready(p);
+  if (p.im_time) {
+    const StackItem& first = p.stack[0];
+    if (first.type != 3 || first.op->kind != OK_LEAK)
+      fail(SO_ERR_STACK, "For imaginary time propagation first control must be LeakControl.");
+    if (!first.op->chk.present)
+      fail(SO_ERR_STACK, "Leak control must have loss checker to get mean energy for imaginary time.");
+    p.flush();
+    p.ctx->set_slot(first.op->chk.s_loss, 0.0);
+    p.run_steps(1);
+    p.flush();
+    const double decay = p.ctx->get_slot(first.op->chk.s_loss);
+    if (first.op->chk.has_saver) {
+      int rc = write_loss_file(first.op);
+      if (rc) return rc;
+    }
+    const double norm = wf_norm_value(p.wf);
+    *out = -std::log(norm - decay) / p.step / 2.;
+  } else {
+    p.flush();
+    so_wave_function *before = nullptr, *after = nullptr;
+    if (so_wave_function_clone(p.wf, &before) != SO_OK) fail(SO_ERR_HIP, "%s", g_err.c_str());
+    p.run_steps(1);
+    p.flush();
+    if (so_wave_function_clone(p.wf, &after) != SO_OK) {
+      wf_free(before);
+      fail(SO_ERR_HIP, "%s", g_err.c_str());
+    }
+    double d[2];
+    int rc = so_wave_function_dot(after, before, d);
+    wf_free(before);
+    wf_free(after);
+    if (rc) return rc;
+    *out = -std::atan2(d[1], d[0]) / p.step;
+  }
+  API_END
+}
+
+extern "C" int so_propagation_get_losses(so_propagation* pr, double* out, int64_t cap, int64_t* n) {
+  API_BEGIN
+  NEED(pr);
+  NEED(n);
+  Propagation& p = pr->p;
+  p.flush();
+  int64_t c = 0;
+  for (auto& it : p.stack)
+    if (it.type == 0 && it.op->chk.present) {
+      double v = 0.0;
+      int present = 0;
+      int rc = so_op_loss(it.op, &v, &present);
+      if (rc) return rc;
+      if (out && c < cap) out[c] = v;
+      c++;
+    }
+  *n = c;
+  API_END
+}
+
+extern "C" int so_propagation_print_losses(so_propagation* pr) {
+  API_BEGIN
+  NEED(pr);
+  Propagation& p = pr->p;
+  p.flush();
+  for (auto& it : p.stack)
+    if ((it.type == 0 || it.type == 3) && it.op->chk.present) {
+      double v = 0.0;
+      int present = 0;
+      int rc = so_op_loss(it.op, &v, &present);
+      if (rc) return rc;
+      printf("%s loss: %.17g\n", it.op->chk.name.c_str(), v);
+    }
+  fflush(stdout);
+  API_END
+}
+
+extern "C" int so_propagation_savers_save(so_propagation* pr) {
+  API_BEGIN
+  NEED(pr);
+  pr->p.flush();
+  for (auto& it : pr->p.stack)
+    if (it.type == 2) {
+      int rc = so_saver_save(it.op);
+      if (rc) return rc;
+    }
+  API_END
+}
+
+extern "C" int so_propagation_reset_savers_state(so_propagation* pr) {
+  API_BEGIN
+  NEED(pr);
+  pr->p.flush();
+  for (auto& it : pr->p.stack)
+    if (it.type == 2) so_saver_reset(it.op);
+  API_END
+}
+
+extern "C" int so_propagation_reset_losses(so_propagation* pr) {
+  API_BEGIN
+  NEED(pr);
+  Propagation& p = pr->p;
+  p.flush();
+  for (auto& it : p.stack)
+    if (it.type == 0 && it.op->chk.present) p.ctx->set_slot(it.op->chk.s_loss, 0.0);
+  API_END
+}
+
+extern "C" int so_propagation_set_graphs(so_propagation* pr, int enable) {
+  API_BEGIN
+  NEED(pr);
+  pr->p.use_graphs = enable != 0;
+  API_END
+}
+
+extern "C" int so_propagation_stats(so_propagation* pr, int64_t* lps, int64_t* compiled) {
+  API_BEGIN
+  NEED(pr);
+  if (lps) *lps = pr->p.last_launches_per_step;
+  if (compiled) *compiled = (int64_t)pr->p.cache.size();
+  API_END
+}
+
+// Times each pass of the last steady-state step on its own: `iters` back-to-back
+// launches between two HIP events on the context stream. Benchmark use only:
+// the repeated launches overwrite the propagation's state.
+extern "C" int so_propagation_time_passes(so_propagation* pr, int64_t iters, int64_t cap, so_pass_timing* out,
+                                          int64_t* n) {
+  API_BEGIN
+  NEED(pr);
+  NEED(n);
+  Propagation& p = pr->p;
+  p.flush();
+  p.ctx->sync();
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int64_t c = 0;
+  for (const auto& L : p.last_steady) {
+    if (L.is_save || L.pass->scalar_only) continue;
+    const sod::PassDesc& h = L.pass->h;
+    for (int64_t w = 0; w < 3; ++w) p.launch(L);
+    HIPCHK(hipEventRecord(e0, p.ctx->stream));
+    for (int64_t i = 0; i < iters; ++i) p.launch(L);
+    HIPCHK(hipEventRecord(e1, p.ctx->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (out && c < cap) {
+      so_pass_timing& t = out[c];
+      memset(&t, 0, sizeof(t));
+      t.avg_ms = (double)ms / (double)iters;
+      t.points = p.wf->size;
+      t.complex_bytes = (int32_t)csize(p.dtype());
+      t.nx = h.nx;
+      t.xform_type = h.nx ? h.xf[0].type : 0;
+      t.axis = h.axis;
+      t.lane = h.n;
+      t.lanes = h.outer * h.inner;
+      t.tiles = h.ntiles;
+      t.lanes_per_tile = h.tl;
+      for (int s = 0; s < 3; ++s)
+        for (int i = 0; i < h.st[s].n; ++i) {
+          const int k = h.st[s].op[i].kind;
+          if (k == sod::EK_MULND) t.nd_tables++;
+          if (k == sod::EK_NORM) t.norms++;
+          if (k == sod::EK_MUL1D || k == sod::EK_MULB || k == sod::EK_SCALE) t.line_ops++;
+        }
+      for (int i = 0; i < h.nx; ++i)
+        if (h.xf[i].type == sod::XF_MAT) {
+          t.mat_flops += (h.xf[i].cplx ? 8.0 : 4.0) * (double)h.n * (double)h.n * (double)(h.outer * h.inner);
+        } else {
+          t.fft_flops += 5.0 * (double)h.n * std::log2((double)h.n) * (double)(h.outer * h.inner);
+        }
+    }
+    c++;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *n = c;
+  API_END
+}
+
+extern "C" int so_propagation_destroy(so_propagation* pr) {
+  API_BEGIN
+  if (!pr) return SO_OK;
+  Propagation& p = pr->p;
+  if (p.ctx) p.ctx->sync();
+  prop_reset_compiled(p);
+  if (p.wf) {
+    p.wf->owner = nullptr;
+    wf_free(p.wf);
+    p.wf = nullptr;
+  }
+  if (pr->stack) so_operation_stack_destroy(pr->stack);
+  delete pr;
+  API_END
+}
+
+extern "C" int so_gauss_legendre(so_context* ctx, int64_t n, double* x, double* w) {
+  API_BEGIN
+  NEED(ctx);
+  NEED(x);
+  NEED(w);
+  if (n < 1) fail(SO_ERR_INVALID, "n must be >= 1");
+  double* d = (double*)ctx->alloc(sizeof(double) * 2 * n);
+  HIPCHK((hipError_t)sod_launch_gauss_legendre(n, d, d + n, ctx->stream));
+  HIPCHK(hipMemcpyAsync(x, d, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(w, d + n, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  ctx->release(d);
+  API_END
+}

@@ -1,0 +1,109 @@
+"""The benchmark workloads, composed from the crate's API.
+
+Each builder takes a backend module `B` that exposes the crate's names
+(split_operator_amd, or the CPU checker wrapper used by the tests) and
+returns a ready Propagation. Random parameters come from SplitMix64 seeded
+with `config_index * 1000 + instance`, generated on the host in f64 so every
+backend sees the same numbers.
+"""
+import math
+
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014): counter-based 64-bit generator."""
+
+    def __init__(self, seed):
+        self.state = seed & MASK64
+
+    def next_u64(self):
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self, lo, hi):
+        return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / (1 << 53)))
+
+
+def harmonic_1d(B, dt=0.01, steps=1000, im_time=False, leak=True, n=1024, x0=2.0, dtype=None):
+    """1-D harmonic trap: 1024 points on [-20, 20], V = x^2/2, hbar = m = omega = 1,
+    Gaussian sigma = 1 at x0 = 2, k0 = 0, stack [LeakControl?, V/2, FFT, T]."""
+    g = B.Grid.new_linear_continuos("x", -20.0, 20.0, n, 0)
+    x = g.nodes
+    arr = np.array([B.gaussian_distribution(xi, x0, 1.0, 0.0) for xi in x])
+    wf = B.WaveFunction(arr, [g]) if dtype is None else B.WaveFunction(arr, [g], dtype=dtype)
+    wf.normalize(1.0)
+    p = B.Propagation()
+    p.set_wave_function(wf)
+    p.set_time_grid(dt, steps, im_time)
+    V = 0.5 * x * x
+    if leak:
+        lc = B.leak_control()
+        lc.add_loss_checker("leak")
+        p.add_control(lc, B.APPLY_FIRST | B.APPLY_SECOND)
+    p.add_propagator(B.one_dim_propagator(B.exp_propagator(V, B.select_step(False, dt, im_time)), 0))
+    p.add_transformation(B.fft_transformation(g, "k"))
+    p.add_propagator(B.one_dim_propagator(B.exp_propagator(B.kinetic_hamiltonian(g, 1.0),
+                                                           B.select_step(True, dt, im_time)), 0))
+    return p, g
+
+
+def radial_angular_params(seed):
+    rng = SplitMix64(seed)
+    c6 = rng.uniform(1.0, 3.0)
+    c12 = rng.uniform(0.5, 2.0)
+    a = rng.uniform(0.0, 0.3)
+    return c6, c12, a
+
+
+def radial_angular(B, n_r=2048, n_theta=128, r_min=0.5, r_max=50.0, dt=0.002, steps=2000, seed=1000,
+                   im_time=False, border=True, leak=False, dtype=None, gl=None, r0=8.0, sigma=1.0):
+    """2-D radial x angular propagation: r in [0.5, 50] (n_r points), theta on n_theta
+    Gauss-Legendre nodes, V = C12/r^12 - C6/r^6 (1 + a P2(cos theta)), m = 1, Legendre
+    transform to l = 0..n_theta-1 with the centrifugal term l(l+1)/(2 m r^2), radial FFT,
+    border damping over the last 10% of r with its loss checked.
+    Stack [BorderDumping?, LeakControl?, V/2, Legendre, Rot/2, FFT_r, T]."""
+    c6, c12, a = radial_angular_params(seed)
+    radial = B.Grid.new_linear_continuos("radial", r_min, r_max, n_r, 0)
+    if gl is None:
+        x, w = np.polynomial.legendre.leggauss(n_theta)
+    else:
+        x, w = gl
+    theta = np.arccos(np.asarray(x))[::-1].copy()
+    wt = np.asarray(w)[::-1].copy()
+    polar = B.Grid.new_custom("polar", theta, wt, 1)
+    r = radial.nodes
+    g_r = np.exp(-(((r - r0) / (2.0 * sigma)) ** 2))
+    psi0 = np.repeat(g_r[:, None], n_theta, axis=1).astype(np.complex128)
+    wf = B.WaveFunction(psi0, [radial, polar]) if dtype is None else B.WaveFunction(psi0, [radial, polar], dtype=dtype)
+    wf.normalize(1.0)
+    p = B.Propagation()
+    p.set_wave_function(wf)
+    p.set_time_grid(dt, steps, im_time)
+    ct = np.cos(theta)
+    p2 = 0.5 * (3.0 * ct * ct - 1.0)
+    V = c12 * r[:, None] ** -12 - c6 * r[:, None] ** -6 * (1.0 + a * p2[None, :])
+    if border:
+        mask = B.dumping_end(0.1 * (r_max - r_min), 0.0, radial)
+        bd = B.border_dumping(mask, radial)
+        bd.add_loss_checker("border")
+        p.add_control(bd, B.APPLY_FIRST | B.APPLY_SECOND)
+    if leak:
+        lc = B.leak_control()
+        lc.add_loss_checker("leak")
+        p.add_control(lc, B.APPLY_FIRST | B.APPLY_SECOND)
+    p.add_propagator(B.n_dim_propagator(B.exp_propagator(V, B.select_step(False, dt, im_time))))
+    t, ti = B.legendre_matrices(polar)
+    l_grid = B.Grid.new_linear_countable("angular_momentum", 0.0, float(n_theta - 1), n_theta, 1)
+    p.add_transformation(B.matrix_transformation(polar, l_grid, t, ti))
+    rot = B.rotational_hamiltonian(radial, polar, 1.0, 0.0)
+    p.add_propagator(B.n_dim_propagator(B.exp_propagator(rot, B.select_step(False, dt, im_time))))
+    p.add_transformation(B.fft_transformation(radial, "radial_momentum"))
+    p.add_propagator(B.one_dim_propagator(B.exp_propagator(B.kinetic_hamiltonian(radial, 1.0),
+                                                           B.select_step(True, dt, im_time)), 0))
+    return p, (radial, polar), (c6, c12, a)

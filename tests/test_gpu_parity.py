@@ -1,0 +1,282 @@
+"""GPU parity: the HIP path against the oracle, the crate's fixtures and known answers.
+
+Tolerances: complex FP64 storage must agree with the f64 oracle to round-off
+(1e-11 relative after hundreds of steps); complex FP32 storage is bounded by
+its own round-off (relative L2 errors of 1e-6 per transform, growing linearly
+with steps; limits are stated per test).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+so = pytest.importorskip("split_operator_amd")
+from oracle import oracle as O  # noqa: E402
+from split_operator_amd import workloads as W  # noqa: E402
+from tests import scenarios as S  # noqa: E402
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+@pytest.fixture(autouse=True)
+def _dtype_reset():
+    yield
+    so.set_default_dtype("f64")
+
+
+# ------------------------------------------------------------------ known answers
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_fft_known_answer_4x4(fixtures, dtype):
+    """tests/fft_tests.rs:12-78 on the GPU: exact transform and bit-exact roundtrip."""
+    a = fixtures["fft4x4_input"]
+    g1 = so.Grid.new_linear_countable("a", 0.0, 1.0, 4, 0)
+    g2 = so.Grid.new_linear_countable("b", 0.0, 1.0, 4, 1)
+    wf = so.WaveFunction(a, [g1, g2], dtype=dtype)
+    fft = so.fft_transformation(g1, "a_fft")
+    fft.transform(wf)
+    transformed = wf.array
+    assert wf.grid(0).name == "a_fft"
+    fft.inverse_transform(wf)
+    assert np.array_equal(transformed, fixtures["fft4x4_expected"])
+    assert np.array_equal(wf.array, a)
+    assert wf.grid(0).name == "a"
+
+
+def test_fft_norm_roundtrip(fixtures):
+    """tests/fft_tests.rs:81-126: the norm survives the roundtrip exactly."""
+    a = fixtures["fft4x4_input"]
+    g1 = so.Grid.new_linear_countable("a", 0.0, 1.0, 4, 0)
+    g2 = so.Grid.new_linear_countable("b", 0.0, 1.0, 4, 1)
+    wf = so.WaveFunction(a, [g1, g2], dtype="f64")
+    fft = so.fft_transformation(g1, "a_fft")
+    n1 = wf.norm()
+    fft.transform(wf)
+    n2 = wf.norm()
+    fft.inverse_transform(wf)
+    n3 = wf.norm()
+    ow = O.WaveFunction(a, [O.Grid.new_linear_countable("a", 0.0, 1.0, 4, 0),
+                            O.Grid.new_linear_countable("b", 0.0, 1.0, 4, 1)])
+    assert n1 == n3
+    assert abs(n1 - ow.norm()) < 1e-13 * n1
+    of = O.fft_transformation(O.Grid.new_linear_countable("a", 0.0, 1.0, 4, 0), "a_fft")
+    of.transform(ow)
+    assert abs(n2 - ow.norm()) < 1e-13 * n2
+
+
+# ------------------------------------------------------------------ lane transforms
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-13), ("f32", 3e-6)])
+@pytest.mark.parametrize("shape,axis", [((512,), 0), ((6, 768), 1), ((1536, 5), 0), ((12, 64, 18), 1),
+                                        ((8, 6, 256), 2), ((256, 3, 4), 0), ((4, 2048), 1), ((2048, 3), 0)])
+def test_fft_lanes_match_numpy(dtype, tol, shape, axis):
+    rng = np.random.default_rng(sum(shape) + axis)
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    grids = [so.Grid.new_linear_continuos(f"g{d}", -1.0, 1.0, n, d) for d, n in enumerate(shape)]
+    wf = so.WaveFunction(a, grids, dtype=dtype)
+    fft = so.fft_transformation(grids[axis], "k")
+    fft.transform(wf)
+    n = shape[axis]
+    ref = np.fft.fft(a, axis=axis) / np.sqrt(n)
+    assert rel(wf.array, ref) < tol
+    fft.inverse_transform(wf)
+    assert rel(wf.array, a) < tol
+
+
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-13), ("f32", 3e-6)])
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("shape,axis", [((40, 32), 1), ((32, 24), 0), ((3, 5, 4), 1), ((100, 128), 1), ((6, 64, 5), 1)])
+def test_matrix_transformation_matches_numpy(dtype, tol, cplx, shape, axis):
+    """matrix_transformation.rs:58-75 on MFMA: lane = M lane along any axis."""
+    rng = np.random.default_rng(len(shape) * 10 + axis + cplx)
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    n = shape[axis]
+    m = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cplx else 0)
+    mi = np.linalg.inv(m)
+    grids = [so.Grid.new_linear_countable(f"g{d}", 0.0, 1.0, s, d) for d, s in enumerate(shape)]
+    lg = so.Grid.new_linear_countable("t", 0.0, 1.0, n, axis)
+    wf = so.WaveFunction(a, grids, dtype=dtype)
+    op = so.matrix_transformation(grids[axis], lg, m, mi.astype(np.complex128))
+    op.transform(wf)
+    ref = np.moveaxis(np.tensordot(m, np.moveaxis(a, axis, 0), axes=(1, 0)), 0, axis)
+    assert rel(wf.array, ref) < tol * np.sqrt(n)
+    assert wf.grid(axis).name == "t"
+    op.inverse_transform(wf)
+    assert rel(wf.array, a) < tol * np.sqrt(n) * np.linalg.cond(m)
+
+
+def test_gauss_legendre_on_gpu():
+    for n in (2, 32, 64, 128):
+        x, w = so.gauss_legendre(n)
+        xr, wr = np.polynomial.legendre.leggauss(n)
+        assert np.max(np.abs(x - xr)) < 1e-14
+        assert np.max(np.abs(w - wr)) < 1e-13
+        xo, wo = O.gauss_legendre(n)
+        assert np.max(np.abs(x - xo)) < 1e-15 and np.max(np.abs(w - wo)) < 1e-14
+
+
+# ------------------------------------------------------------------ crate fixtures
+@pytest.mark.parametrize("im_time,name", [(False, "harmonic"), (True, "harmonic_imaginary")])
+def test_harmonic_fixture_f64(fixtures, im_time, name):
+    so.set_default_dtype("f64")
+    p, sv, _ = S.harmonic_trap(so, im_time, name)
+    p.propagate()
+    data, times = sv.frames(50)
+    assert np.array_equal(times, fixtures[name + "_time"])
+    cols = fixtures[name + "__cols"]
+    assert rel(data[:, cols], fixtures[name + "__frames"]) < 1e-11
+    po, svo, _ = S.harmonic_trap(O, im_time, name)
+    po.propagate()
+    assert rel_l2(p.wave_function().array, po.wave_function().array) < 1e-11
+
+
+def test_lj_ground_fixture_f64(fixtures):
+    so.set_default_dtype("f64")
+    p, sv, _ = S.lj_ground(so)
+    p.propagate()
+    data, times = sv.frames(120)
+    assert np.array_equal(times, fixtures["lj_ground_space_time"])
+    cols = fixtures["lj_ground_space__cols"]
+    assert rel(data[:, cols], fixtures["lj_ground_space__frames"]) < 1e-11
+    po, _, _ = S.lj_ground(O)
+    po.propagate()
+    e, eo = p.mean_energy(), po.mean_energy()
+    assert abs(e - eo) < 1e-10 * abs(eo)
+
+
+def test_harmonic_fixture_f32(fixtures):
+    so.set_default_dtype("f32")
+    p, sv, _ = S.harmonic_trap(so, False, "harmonic")
+    p.propagate()
+    data, _ = sv.frames(50)
+    cols = fixtures["harmonic__cols"]
+    assert rel(data[:, cols], fixtures["harmonic__frames"]) < 5e-4
+
+
+# ------------------------------------------------------------------ workloads vs oracle
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-10), ("f32", 2e-5)])
+def test_harmonic_1d_1000_steps(dtype, tol):
+    """1-D harmonic trap, 1024 points, dt = 0.01, 1000 steps with FP64 LeakControl."""
+    p, g = W.harmonic_1d(so, dtype=dtype)
+    p.propagate()
+    po, _ = W.harmonic_1d(O)
+    po.propagate()
+    err = rel_l2(p.wave_function().array, po.wave_function().array)
+    assert err < tol, err
+    # the Strang leapfrog closed form for <x>
+    psi = p.wave_function().array
+    rho = np.abs(psi) ** 2 * g.weights
+    mean_x = np.sum(rho * g.nodes) / np.sum(rho)
+    assert abs(mean_x - 2.0 * np.cos(1000 * np.arccos(1 - 0.01 ** 2 / 2))) < (1e-9 if dtype == "f64" else 1e-4)
+
+
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-10), ("f32", 5e-5)])
+def test_radial_angular_small_vs_oracle(dtype, tol):
+    """The 2-D radial x angular stack at reduced size (256 x 32, 60 steps) vs the oracle,
+    including the border loss (loss_checker.rs:54-63)."""
+    kw = dict(n_r=256, n_theta=32, steps=60, dt=0.002)
+    p, _, _ = W.radial_angular(so, dtype=dtype, **kw)
+    p.propagate()
+    po, _, _ = W.radial_angular(O, **kw)
+    po.propagate()
+    err = rel_l2(p.wave_function().array, po.wave_function().array)
+    assert err < tol, err
+    # BorderDumping is a Control: its loss is read from the op, as print_losses does
+    assert p.get_losses() == [] and po.get_losses() == []
+
+
+def test_radial_angular_border_loss_and_leak():
+    """Border loss of the radial x angular stack with a packet launched at the border."""
+    kw = dict(n_r=256, n_theta=32, steps=80, dt=0.01, r0=45.0, sigma=1.0, leak=True)
+    p, _, _ = W.radial_angular(so, dtype="f64", **kw)
+    p.propagate()
+    po, _, _ = W.radial_angular(O, **kw)
+    po.propagate()
+    assert rel_l2(p.wave_function().array, po.wave_function().array) < 1e-10
+    assert abs(p.wave_function().norm() - po.wave_function().norm()) < 1e-12
+
+
+def test_graphs_replay_matches_eager():
+    """hipGraph replay of steady-state steps gives the same state as eager launches."""
+    out = []
+    for graphs in (False, True):
+        p, _, _ = W.radial_angular(so, dtype="f32", n_r=512, n_theta=64, steps=300)
+        p._ready()
+        p.set_graphs(graphs)
+        p.propagate()
+        out.append(p.wave_function().array)
+    assert np.array_equal(out[0], out[1])
+
+
+def test_imaginary_time_mean_energy_f64():
+    dt = 0.01
+    p, _ = W.harmonic_1d(so, dt=dt, steps=1500, im_time=True, dtype="f64")
+    p.propagate()
+    e = p.mean_energy()
+    assert abs(e - np.arccosh(1 + dt * dt / 2) / (2 * dt)) < 1e-9
+
+
+def _saver_stack(B, dtype=None):
+    """[StateSaver(First), WaveFunctionSaver(Second), BorderDumping(+LossSaver), V/2,
+    Legendre, Rot/2, FFT_r, T] on a 128 x 16 radial x angular grid."""
+    steps, dt = 40, 0.01
+    radial = B.Grid.new_linear_continuos("radial", 0.5, 50.0, 128, 0)
+    x, w = np.polynomial.legendre.leggauss(16)
+    polar = B.Grid.new_custom("polar", np.arccos(x)[::-1].copy(), w[::-1].copy(), 1)
+    r = radial.nodes
+    psi0 = np.repeat(np.exp(-(((r - 45.0) / 2.0) ** 2))[:, None], 16, axis=1).astype(np.complex128)
+    wf = B.WaveFunction(psi0, [radial, polar]) if dtype is None else B.WaveFunction(psi0, [radial, polar], dtype=dtype)
+    wf.normalize(1.0)
+    p = B.Propagation()
+    p.set_wave_function(wf)
+    p.set_time_grid(dt, steps, False)
+    sv = B.state_saver("st", dt, steps, polar, 8)
+    wsv = B.wave_function_saver("wf2d", dt, steps, radial, polar, 4)
+    bd = B.border_dumping(B.dumping_end(10.0, 1.0, radial), radial)
+    bd.add_loss_checker("border", 10, dt, steps)
+    p.add_saver(sv, B.APPLY_FIRST)
+    p.add_saver(wsv, B.APPLY_SECOND)
+    p.add_control(bd, B.APPLY_FIRST | B.APPLY_SECOND)
+    V = -1.0 / r[:, None] * np.ones((1, 16))
+    p.add_propagator(B.n_dim_propagator(B.exp_propagator(V, B.select_step(False, dt, False))))
+    t, ti = B.legendre_matrices(polar)
+    lg = B.Grid.new_linear_countable("angular_momentum", 0.0, 15.0, 16, 1)
+    p.add_transformation(B.matrix_transformation(polar, lg, t, ti))
+    rot = B.rotational_hamiltonian(radial, polar, 1.0, 0.0)
+    p.add_propagator(B.n_dim_propagator(B.exp_propagator(rot, B.select_step(False, dt, False))))
+    p.add_transformation(B.fft_transformation(radial, "k"))
+    p.add_propagator(B.one_dim_propagator(B.exp_propagator(B.kinetic_hamiltonian(radial, 1.0),
+                                                           B.select_step(True, dt, False)), 0))
+    return p, sv, wsv, bd
+
+
+def test_savers_and_loss_saver(tmp_path, monkeypatch):
+    """StateSaver (N-D marginal), WaveFunctionSaver (2-D density) and LossSaver records vs the oracle."""
+    monkeypatch.chdir(tmp_path)
+    p, sv, wsv, bd = _saver_stack(so, dtype="f64")
+    p.propagate()
+    po, svo, wsvo, bdo = _saver_stack(O)
+    po.propagate()
+    d1, t1 = sv.frames(8)
+    d2, t2 = svo.frames(8)
+    w1, wt1 = wsv.frames(4)
+    w2, wt2 = wsvo.frames(4)
+    assert np.array_equal(t1, t2) and np.array_equal(wt1, wt2)
+    assert rel(d1, d2) < 1e-10 and rel(w1, w2) < 1e-10
+    assert abs(bd.loss() - bdo.loss()) < 1e-12
+    lt1, l1 = bd.loss_frames()
+    lt2, l2 = bdo.loss_frames()
+    assert np.array_equal(lt1, lt2) and np.max(np.abs(l1 - l2)) < 1e-12
+    assert bd.loss() > 1e-4  # the packet reaches the mask
+    # Saver::save writes the crate's files
+    p.savers_save()
+    assert (tmp_path / "st.npy").exists() and (tmp_path / "st_polar_grid.npy").exists()
+    assert (tmp_path / "wf2d_x_grid.npy").exists() and (tmp_path / "border.dat").exists()
+    saved = np.load(tmp_path / "st.npy", allow_pickle=False)
+    assert saved.shape == (16, 8) and np.array_equal(saved, d1)
