@@ -49,6 +49,7 @@ def lib():
             "orc_wf_free": (None, [vp]),
             "orc_wf_data": (_dp, [vp]),
             "orc_wf_size": (C.c_int64, [vp]),
+            "orc_wf_shape": (C.c_int, [vp, _i64p]),
             "orc_wf_grid": (vp, [vp, C.c_int]),
             "orc_wf_norm": (C.c_double, [vp]),
             "orc_wf_normalize": (None, [vp, C.c_double]),
@@ -187,7 +188,10 @@ class WaveFunction:
     @property
     def array(self):
         n = lib().orc_wf_size(self._h)
-        return np.ctypeslib.as_array(lib().orc_wf_data(self._h), shape=(2 * n,)).copy().view(np.complex128)
+        shape = np.zeros(8, dtype=np.int64)
+        nd = lib().orc_wf_shape(self._h, shape.ctypes.data_as(_i64p))
+        a = np.ctypeslib.as_array(lib().orc_wf_data(self._h), shape=(2 * n,)).copy().view(np.complex128)
+        return a.reshape(tuple(shape[:nd]))
 
     def grid(self, axis):
         return Grid(lib().orc_wf_grid(self._h, axis), owned=False)

@@ -244,6 +244,10 @@ void orc_wf_free(orc_wf *wf) {
 }
 double *orc_wf_data(orc_wf *wf) { return (double *)wf->a; }
 int64_t orc_wf_size(const orc_wf *wf) { return wf->size; }
+int orc_wf_shape(const orc_wf *wf, int64_t *shape) {
+    for (int i = 0; i < wf->ndim; i++) shape[i] = wf->shape[i];
+    return wf->ndim;
+}
 const orc_grid *orc_wf_grid(const orc_wf *wf, int axis) { return wf->grids[axis]; }
 
 /* wave_function.rs:68-85 */

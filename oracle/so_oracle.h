@@ -49,6 +49,7 @@ orc_wf *orc_wf_clone(const orc_wf *wf);
 void orc_wf_free(orc_wf *wf);
 double *orc_wf_data(orc_wf *wf);
 int64_t orc_wf_size(const orc_wf *wf);
+int orc_wf_shape(const orc_wf *wf, int64_t *shape);
 const orc_grid *orc_wf_grid(const orc_wf *wf, int axis);
 double orc_wf_norm(orc_wf *wf);
 void orc_wf_normalize(orc_wf *wf, double new_norm);

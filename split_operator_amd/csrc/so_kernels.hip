@@ -231,12 +231,16 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 template <typename T>
 struct Mfma;
+// Output rows held by lane group lk = L/16 in accumulator j: 4 lk + j for the
+// f32 instruction, lk + 4 j for the f64 one (measured on gfx950 with exact
+// small-integer operands).
 template <>
 struct Mfma<float> {
   using acc_t = v4f;
   static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int j) { return 4 * lk + j; }
 };
 template <>
 struct Mfma<double> {
@@ -244,13 +248,14 @@ struct Mfma<double> {
   static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int j) { return lk + 4 * j; }
 };
 
 // out-lanes = M . in-lanes for `tl` lanes in LDS. Real matrix: GEMM rows are
 // (lane, re|im) pairs, K = n, columns = n. Complex matrix: rows are lanes seen
 // as 2n reals, K = 2n, columns = 2n, B holds the real 2n x 2n form of M^T.
 // 16x16x4 MFMA layout: lane L supplies A[L%16][L/16] and B[L/16][L%16] and
-// holds D[4(L/16)+j][L%16], j = 0..3.
+// holds D[Mfma<T>::row(L/16, j)][L%16], j = 0..3.
 template <typename T, typename T2>
 __device__ void mat_lds(const T2* in, T2* out, int tl, int ld, const XformDesc& x, int tid) {
   using M = Mfma<T>;
@@ -288,7 +293,7 @@ __device__ void mat_lds(const T2* in, T2* out, int tl, int ld, const XformDesc& 
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int r = rt * 16 + 4 * lk + j;
+      const int r = rt * 16 + M::row(lk, j);
       const int col = ct * 16 + lr;
       if (r < rows && col < ncols) {
         if (x.cplx)
