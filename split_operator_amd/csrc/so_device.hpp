@@ -21,8 +21,12 @@ constexpr int MAX_SCALAR = 32;  // scalar instructions per pass
 constexpr int MAX_RADIX = 16;   // radix passes per FFT
 constexpr int MAX_TL = 64;      // lanes per workgroup tile
 constexpr int NTHREADS = 256;   // threads per workgroup (4 wave64)
-constexpr int FFT_EPT = 32;     // complex elements per thread held in registers by an FFT radix pass
-#define SOD_MAX_DYN_LDS (64 * 1024)  // dynamic LDS bytes a pass may request per workgroup
+constexpr int MAX_LDS_ARR = 4;  // small per-pass arrays staged in LDS at kernel start
+constexpr int MAX_PF = 3;       // full-grid tables prefetched into registers at kernel start
+constexpr int FFT_EPT = 16;     // max complex elements per thread of a transform tile (tl * n <= FFT_EPT * NTHREADS)
+#define SOD_MAX_DYN_LDS (152 * 1024)  // dynamic LDS bytes a pass may request per workgroup (gfx950: 160 KiB)
+#define SOD_PREF_DYN_LDS (64 * 1024)  // tiles are sized to this when a single lane fits, two workgroups per CU
+#define SOD_ELEM_EPT 8               // elements per thread of an element-wise pass
 
 enum ElemKind : int32_t {
   EK_MUL1D = 1,  // z *= table[i_axis]          (OneDimPropagator, BorderDumping mask)
@@ -39,6 +43,8 @@ struct ElemOp {
   int32_t slot;   // EK_SCALE: scalar arena slot
   const void* table;
   const double* w_axis;  // EK_NORM: weights of the pass axis for the grid current at that point
+  int32_t lds;    // >= 0: the table (EK_MUL1D on the pass axis) or weights (EK_NORM) are staged in LDS array `lds`
+  int32_t pf;     // >= 0: EK_MULND values prefetched into register set `pf` at kernel start
 };
 
 struct Stage {
@@ -98,6 +104,8 @@ struct PassDesc {
   int64_t tiles_per_outer;  // inner > 1: tiles per outer index
   int32_t nx;        // number of transforms (0, 1, 2)
   int32_t nacc;      // number of norm accumulators in use
+  int32_t ept;       // elements per thread (transform passes: 4, 8, 16 or 32)
+  int32_t pad4;
   XformDesc xf[2];
   Stage st[3];       // st[0] before xf[0]; st[1] after xf[0]; st[2] after xf[1]
   const double* w_other[MAXD];  // weights of the other axes (grids fixed during the pass)
@@ -107,9 +115,28 @@ struct PassDesc {
   unsigned int* counter;
   double* S;         // scalar arena
   int32_t nonfinite_slot;
-  int32_t pad3;
+  int32_t fin_mode;  // FIN_NONE / FIN_IMMEDIATE (last workgroup) / FIN_DEFERRED (next pass runs the program)
   ScalarOp sc[MAX_SCALAR];
+  // small arrays copied into LDS at kernel start (1-D tables along the pass axis, norm weights)
+  int32_t n_lds_arr;
+  int32_t lds_arr_bytes[MAX_LDS_ARR];
+  int32_t lds_arr_off[MAX_LDS_ARR];  // byte offsets in the dynamic LDS segment
+  const void* lds_arr_src[MAX_LDS_ARR];
+  // full-grid tables whose elements are loaded into registers at kernel start
+  int32_t n_pf;
+  int32_t pad5;
+  const void* pf_src[MAX_PF];
+  // scalar program inherited from the previous pass (run by workgroup 0 at start)
+  int32_t prev_nscalar;
+  int32_t pad6;
+  const double* prev_partials;
+  int64_t prev_ntiles;
+  ScalarOp prev_sc[MAX_SCALAR];
+  int32_t lds_bytes;  // total dynamic LDS of the launch
+  int32_t pad7;
 };
+
+enum FinMode : int32_t { FIN_NONE = 0, FIN_IMMEDIATE = 1, FIN_DEFERRED = 2 };
 
 }  // namespace sod
 
@@ -118,6 +145,7 @@ struct PassDesc {
 extern "C++" {
 int sod_launch_pass(int dtype, const sod::PassDesc* d_desc, const sod::PassDesc& h_desc, hipStream_t s);
 int sod_launch_scalar(const sod::PassDesc* d_desc, hipStream_t s);
+int sod_pass_lds_bytes(int dtype, const sod::PassDesc& h);
 int sod_launch_state_density(int dtype, const void* psi, int ndim, const int64_t* shape, int axis,
                              const double* const* w, double* out, hipStream_t s);
 int sod_launch_density(int dtype, const void* psi, int64_t size, double* out, int64_t out_stride, hipStream_t s);

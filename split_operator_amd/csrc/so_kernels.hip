@@ -21,6 +21,60 @@ using namespace sod;
 
 namespace {
 
+// Pointers read from a pass descriptor are generic; these casts make the
+// accesses global_load/global_store instead of flat ones.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SOD_GLOBAL __attribute__((address_space(1)))
+#define SOD_LOCAL __attribute__((address_space(3)))
+#else
+#define SOD_GLOBAL
+#define SOD_LOCAL
+#endif
+// Class types (float2, double2, uint2) are moved as raw 8/16-byte vectors so
+// the address space survives (a class copy would go through a generic reference).
+typedef unsigned int sod_u4 __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ T gld(const T* p) {
+  if constexpr (sizeof(T) == 16) {
+    return __builtin_bit_cast(T, *(const SOD_GLOBAL sod_u4*)p);
+  } else if constexpr (sizeof(T) == 8) {
+    return __builtin_bit_cast(T, *(const SOD_GLOBAL unsigned long long*)p);
+  } else {
+    return __builtin_bit_cast(T, *(const SOD_GLOBAL unsigned int*)p);
+  }
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, T v) {
+  if constexpr (sizeof(T) == 16) {
+    *(SOD_GLOBAL sod_u4*)p = __builtin_bit_cast(sod_u4, v);
+  } else if constexpr (sizeof(T) == 8) {
+    *(SOD_GLOBAL unsigned long long*)p = __builtin_bit_cast(unsigned long long, v);
+  } else {
+    *(SOD_GLOBAL unsigned int*)p = __builtin_bit_cast(unsigned int, v);
+  }
+}
+// LDS element access through pointers the compiler cannot trace to __shared__
+template <typename T>
+__device__ __forceinline__ T lld(const T* p) {
+  if constexpr (sizeof(T) == 16) {
+    return __builtin_bit_cast(T, *(const SOD_LOCAL sod_u4*)p);
+  } else if constexpr (sizeof(T) == 8) {
+    return __builtin_bit_cast(T, *(const SOD_LOCAL unsigned long long*)p);
+  } else {
+    return __builtin_bit_cast(T, *(const SOD_LOCAL unsigned int*)p);
+  }
+}
+template <typename T>
+__device__ __forceinline__ void lst(T* p, T v) {
+  if constexpr (sizeof(T) == 16) {
+    *(SOD_LOCAL sod_u4*)p = __builtin_bit_cast(sod_u4, v);
+  } else if constexpr (sizeof(T) == 8) {
+    *(SOD_LOCAL unsigned long long*)p = __builtin_bit_cast(unsigned long long, v);
+  } else {
+    *(SOD_LOCAL unsigned int*)p = __builtin_bit_cast(unsigned int, v);
+  }
+}
+
 template <typename T>
 struct CT;
 template <>
@@ -53,60 +107,51 @@ __device__ __forceinline__ T2 csub(T2 a, T2 b) {
   r.y = a.y - b.y;
   return r;
 }
-// multiply by exp(-i pi/2) = -i (forward) or +i (inverse)
-template <bool INV, typename T2>
-__device__ __forceinline__ T2 rot90(T2 z) {
+// multiply by exp(-i pi/2) = -i (forward, sg = 1) or +i (inverse, sg = -1)
+template <typename T, typename T2>
+__device__ __forceinline__ T2 rot90(T2 z, T sg) {
   T2 r;
-  if (INV) {
-    r.x = -z.y;
-    r.y = z.x;
-  } else {
-    r.x = z.y;
-    r.y = -z.x;
-  }
+  r.x = sg * z.y;
+  r.y = -sg * z.x;
   return r;
 }
 
 // ---------------------------------------------------------------- small DFTs
-template <bool INV, typename T2>
-__device__ __forceinline__ void dft2(T2* v) {
+// sg = +1: forward (exp(-i ...)), sg = -1: inverse
+template <typename T, typename T2>
+__device__ __forceinline__ void dft2(T2* v, T) {
   T2 a = v[0], b = v[1];
   v[0] = cadd(a, b);
   v[1] = csub(a, b);
 }
 
-template <bool INV, typename T2>
-__device__ __forceinline__ void dft4(T2* v) {
+template <typename T, typename T2>
+__device__ __forceinline__ void dft4(T2* v, T sg) {
   T2 t0 = cadd(v[0], v[2]);
   T2 t1 = csub(v[0], v[2]);
   T2 t2 = cadd(v[1], v[3]);
-  T2 t3 = rot90<INV>(csub(v[1], v[3]));
+  T2 t3 = rot90(csub(v[1], v[3]), sg);
   v[0] = cadd(t0, t2);
   v[2] = csub(t0, t2);
   v[1] = cadd(t1, t3);
   v[3] = csub(t1, t3);
 }
 
-template <bool INV, typename T, typename T2>
-__device__ __forceinline__ void dft8(T2* v) {
+template <typename T, typename T2>
+__device__ __forceinline__ void dft8(T2* v, T sg) {
   const T c = (T)0.70710678118654752440084436210485;
   T2 e[4] = {v[0], v[2], v[4], v[6]};
   T2 o[4] = {v[1], v[3], v[5], v[7]};
-  dft4<INV>(e);
-  dft4<INV>(o);
+  dft4(e, sg);
+  dft4(o, sg);
+  // o1 *= exp(-sg i pi/4), o3 *= exp(-sg 3 i pi/4)
+  const T cs = c * sg;
   T2 o1, o3;
-  if (INV) {  // w8 = exp(+i pi/4)
-    o1.x = c * (o[1].x - o[1].y);
-    o1.y = c * (o[1].x + o[1].y);
-    o3.x = -c * (o[3].x + o[3].y);
-    o3.y = c * (o[3].x - o[3].y);
-  } else {  // w8 = exp(-i pi/4)
-    o1.x = c * (o[1].x + o[1].y);
-    o1.y = c * (o[1].y - o[1].x);
-    o3.x = c * (o[3].y - o[3].x);
-    o3.y = -c * (o[3].x + o[3].y);
-  }
-  T2 o2 = rot90<INV>(o[2]);
+  o1.x = c * o[1].x + cs * o[1].y;
+  o1.y = c * o[1].y - cs * o[1].x;
+  o3.x = -c * o[3].x + cs * o[3].y;
+  o3.y = -c * o[3].y - cs * o[3].x;
+  T2 o2 = rot90(o[2], sg);
   v[0] = cadd(e[0], o[0]);
   v[4] = csub(e[0], o[0]);
   v[1] = cadd(e[1], o1);
@@ -117,8 +162,8 @@ __device__ __forceinline__ void dft8(T2* v) {
   v[7] = csub(e[3], o3);
 }
 
-template <bool INV, typename T, typename T2>
-__device__ __forceinline__ void dft3(T2* v) {
+template <typename T, typename T2>
+__device__ __forceinline__ void dft3(T2* v, T sg) {
   const T s = (T)0.86602540378443864676372317075294;  // sin(2 pi / 3)
   T2 t = cadd(v[1], v[2]);
   T2 u = csub(v[1], v[2]);
@@ -128,51 +173,75 @@ __device__ __forceinline__ void dft3(T2* v) {
   T2 su;
   su.x = s * u.x;
   su.y = s * u.y;
-  T2 r = rot90<INV>(su);
+  T2 r = rot90(su, sg);
   v[0] = cadd(v[0], t);
   v[1] = cadd(m, r);
   v[2] = csub(m, r);
 }
 
-template <int R, bool INV, typename T, typename T2>
-__device__ __forceinline__ void dft(T2* v) {
-  if constexpr (R == 2) dft2<INV>(v);
-  if constexpr (R == 3) dft3<INV, T>(v);
-  if constexpr (R == 4) dft4<INV>(v);
-  if constexpr (R == 8) dft8<INV, T>(v);
+template <int R, typename T, typename T2>
+__device__ __forceinline__ void dft(T2* v, T sg) {
+  if constexpr (R == 2) dft2(v, sg);
+  if constexpr (R == 3) dft3(v, sg);
+  if constexpr (R == 4) dft4(v, sg);
+  if constexpr (R == 8) dft8(v, sg);
 }
 
+// LDS index of lane element i in FFT passes: one pad slot every 32 complex
+// elements breaks the power-of-two strides of the Stockham reads and writes
+// across the 64 LDS banks.
+__device__ __forceinline__ int fpad(int i) { return i + (i >> 5); }
+
 // One Stockham radix-R pass over `tl` lanes of length n held in LDS (row pitch
-// ld). Butterfly j of a lane reads x[j + r n/R], twiddles by W_{Ls R}^{r k}
-// (k = j mod Ls), and writes y[(j/Ls) Ls R + k + r Ls]. All reads land in
-// registers before the barrier, so the pass is done in place.
-template <int R, bool INV, typename T, typename T2>
-__device__ void radix_pass(T2* lds, int tl, int n, int ld, int Ls, const T2* __restrict__ tw, T scale,
-                           bool last, int tid) {
-  constexpr int MAXB = FFT_EPT / R;
+// ld, padded indices). Butterfly j of a lane reads x[j + r n/R], twiddles by
+// W_{Ls R}^{r k} (k = j mod Ls) and writes y[(j/Ls) Ls R + k + r Ls]. The
+// twiddles of the pass are laid out as tw[(r-1) Ls + k], so the lanes of a
+// wave read consecutive entries. All reads land in registers before the
+// barrier, so the pass is done in place.
+template <int R, int EPT, typename T, typename T2>
+__device__ __forceinline__ void radix_pass(T2* lds, int tl, int n, int ld, int Ls, const T2* tw, T scale, T sg,
+                                           bool last, int tid) {
+  constexpr int MAXB = EPT / R > 0 ? EPT / R : 1;
   const int nb = n / R;
   const int total = tl * nb;
-  const int twstep = n / (Ls * R);
+  // power-of-two strides (every pass before the radix-3 ones) use shifts
+  const bool pow2 = ((Ls & (Ls - 1)) == 0) && ((nb & (nb - 1)) == 0);
+  const int ls_sh = __builtin_ctz((unsigned)Ls);
+  const int nb_sh = __builtin_ctz((unsigned)nb);
   T2 v[MAXB][R];
+  int dsti[MAXB], drow[MAXB];
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
     const int b = tid + q * NTHREADS;
+    dsti[q] = 0;
+    drow[q] = 0;
     if (b < total) {
-      const int l = b / nb;
-      const int j = b - l * nb;
-      const int k = j % Ls;
+      int l, j, k, jd;
+      if (pow2) {
+        l = b >> nb_sh;
+        j = b & (nb - 1);
+        k = j & (Ls - 1);
+        jd = j >> ls_sh;
+      } else {
+        l = b / nb;
+        j = b - l * nb;
+        jd = j / Ls;
+        k = j - jd * Ls;
+      }
       const T2* src = lds + l * ld;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = src[j + r * nb];
+      for (int r = 0; r < R; ++r) v[q][r] = lld(&src[fpad(j + r * nb)]);
       if (Ls > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          T2 w = tw[r * k * twstep];
-          if (INV) w.y = -w.y;
+          T2 w = lld(&tw[(r - 1) * Ls + k]);
+          w.y *= sg;
           v[q][r] = cmul(v[q][r], w);
         }
       }
-      dft<R, INV, T>(v[q]);
+      dft<R>(v[q], sg);
+      drow[q] = l * ld;
+      dsti[q] = jd * Ls * R + k;
     }
   }
   __syncthreads();
@@ -180,10 +249,7 @@ __device__ void radix_pass(T2* lds, int tl, int n, int ld, int Ls, const T2* __r
   for (int q = 0; q < MAXB; ++q) {
     const int b = tid + q * NTHREADS;
     if (b < total) {
-      const int l = b / nb;
-      const int j = b - l * nb;
-      const int k = j % Ls;
-      T2* dst = lds + l * ld + (j / Ls) * Ls * R + k;
+      T2* dst = lds + drow[q];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         T2 z = v[q][r];
@@ -191,38 +257,34 @@ __device__ void radix_pass(T2* lds, int tl, int n, int ld, int Ls, const T2* __r
           z.x *= scale;
           z.y *= scale;
         }
-        dst[r * Ls] = z;
+        lst(&dst[fpad(dsti[q] + r * Ls)], z);
       }
     }
   }
   __syncthreads();
 }
 
-template <bool INV, typename T, typename T2>
-__device__ void fft_lds_dir(T2* lds, int tl, int n, int ld, const XformDesc& x, int tid) {
-  const T2* tw = (const T2*)x.tw;
-  const T scale = (T)x.scale;
+template <int EPT, typename T, typename T2>
+__device__ __forceinline__ void fft_lds(T2* lds, int tl, int n, int ld, const T2* tw, T scale, bool inverse, int nrad,
+                                        const int32_t* rad, int tid) {
+  const T sg = inverse ? (T)-1 : (T)1;
   int Ls = 1;
-  const int nrad = x.nrad;
+  int off = 0;  // start of the current pass's twiddles
   for (int p = 0; p < nrad; ++p) {
-    const int R = x.rad[p];
+    const int R = rad[p];
     const bool last = (p == nrad - 1);
-    switch (R) {
-      case 8: radix_pass<8, INV, T>(lds, tl, n, ld, Ls, tw, scale, last, tid); break;
-      case 4: radix_pass<4, INV, T>(lds, tl, n, ld, Ls, tw, scale, last, tid); break;
-      case 3: radix_pass<3, INV, T>(lds, tl, n, ld, Ls, tw, scale, last, tid); break;
-      default: radix_pass<2, INV, T>(lds, tl, n, ld, Ls, tw, scale, last, tid); break;
-    }
+    const T2* twp = tw + off;
+    if (R == 8)
+      radix_pass<8, EPT>(lds, tl, n, ld, Ls, twp, scale, sg, last, tid);
+    else if (R == 4)
+      radix_pass<4, EPT>(lds, tl, n, ld, Ls, twp, scale, sg, last, tid);
+    else if (R == 3)
+      radix_pass<3, EPT>(lds, tl, n, ld, Ls, twp, scale, sg, last, tid);
+    else
+      radix_pass<2, EPT>(lds, tl, n, ld, Ls, twp, scale, sg, last, tid);
+    off += (R - 1) * Ls;
     Ls *= R;
   }
-}
-
-template <typename T, typename T2>
-__device__ void fft_lds(T2* lds, int tl, int n, int ld, const XformDesc& x, int tid) {
-  if (x.inverse)
-    fft_lds_dir<true, T>(lds, tl, n, ld, x, tid);
-  else
-    fft_lds_dir<false, T>(lds, tl, n, ld, x, tid);
 }
 
 // ---------------------------------------------------------------- MFMA lane matrix product
@@ -256,12 +318,78 @@ struct Mfma<double> {
 // as 2n reals, K = 2n, columns = 2n, B holds the real 2n x 2n form of M^T.
 // 16x16x4 MFMA layout: lane L supplies A[L%16][L/16] and B[L/16][L%16] and
 // holds D[Mfma<T>::row(L/16, j)][L%16], j = 0..3.
+template <typename T>
+struct KChunk {
+  static constexpr int value = sizeof(T) == 4 ? 16 : 8;  // MFMA k-steps whose operands are held in registers
+};
+template <typename T>
+struct BFast {
+  static constexpr int KS = sizeof(T) == 4 ? 32 : 16;  // k-steps of the prefetched B fragment
+  static constexpr int TPW = 2;                         // 16x16 output tiles per wave
+};
+
+// B fragments of one GEMM for the tiles a wave owns, loaded into registers
+// ahead of use (fast path: at most TPW tiles per wave and kpad <= 4 KS).
+template <typename T>
+struct BRegs {
+  T b[BFast<T>::TPW][BFast<T>::KS];
+};
+
+template <typename T>
+__device__ __forceinline__ bool mat_fast(const XformDesc& x, int tl) {
+  const int rows = x.cplx ? tl : 2 * tl;
+  const int tiles = ((rows + 15) / 16) * (x.npad / 16);
+  return tiles <= BFast<T>::TPW * (NTHREADS / 64) && x.kpad <= 4 * BFast<T>::KS;
+}
+
+template <typename T>
+__device__ __forceinline__ void mat_prefetch(const XformDesc& x, int tl, int tid, BRegs<T>& br) {
+  const int rows = x.cplx ? tl : 2 * tl;
+  const int rtiles = (rows + 15) / 16;
+  const int ctiles = x.npad / 16;
+  const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const T* __restrict__ B = (const T*)x.B;
+#pragma unroll
+  for (int u = 0; u < BFast<T>::TPW; ++u) {
+    const int t = wave + u * (NTHREADS / 64);
+    const int ct = t < rtiles * ctiles ? t % ctiles : 0;
+    const T* bcol = B + ct * 16 + lr;
+#pragma unroll
+    for (int c = 0; c < BFast<T>::KS; ++c) {
+      const int k = 4 * c + lk;
+      br.b[u][c] = (t < rtiles * ctiles && k < x.kpad) ? gld(&bcol[(int64_t)k * x.npad]) : (T)0;
+    }
+  }
+}
+
+template <typename T, typename T2, typename ACC>
+__device__ __forceinline__ void mat_store_tile(T* Cm, const ACC& acc, int rt, int ct, int rows, int ncols, int ld,
+                                               bool cplx, int lr, int lk) {
+  using M = Mfma<T>;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int r = rt * 16 + M::row(lk, j);
+    const int col = ct * 16 + lr;
+    if (r < rows && col < ncols) {
+      if (cplx)
+        lst(&Cm[r * ld * 2 + col], (T)acc[j]);
+      else
+        lst(&Cm[((r >> 1) * ld + col) * 2 + (r & 1)], (T)acc[j]);
+    }
+  }
+}
+
+// out-lanes = M . in-lanes for `tl` lanes in LDS. Real matrix: GEMM rows are
+// (lane, re|im) pairs, K = n, columns = n. Complex matrix: rows are lanes seen
+// as 2n reals, K = 2n, columns = 2n, B holds the real 2n x 2n form of M^T.
+// Each wave owns 16 x 16 output tiles. Fast path: B was prefetched into
+// registers; otherwise B and A are loaded per chunk of KC k-steps.
 template <typename T, typename T2>
-__device__ void mat_lds(const T2* in, T2* out, int tl, int ld, const XformDesc& x, int tid) {
+__device__ __forceinline__ void mat_lds(const T2* in, T2* out, int tl, int ld, const XformDesc& x, int tid,
+                                        const BRegs<T>& br, bool fast) {
   using M = Mfma<T>;
   const T* A = (const T*)in;
   T* Cm = (T*)out;
-  const T* __restrict__ B = (const T*)x.B;
   const int rows = x.cplx ? tl : 2 * tl;
   const int rtiles = (rows + 15) / 16;
   const int ctiles = x.npad / 16;
@@ -270,38 +398,71 @@ __device__ void mat_lds(const T2* in, T2* out, int tl, int ld, const XformDesc& 
   const int lr = lane & 15;
   const int lk = lane >> 4;
   const int kdim = x.kdim, kpad = x.kpad, npad = x.npad, ncols = x.ncols;
+  if (fast) {
+    // the (up to) two tiles of this wave run interleaved MFMA chains
+    const int t0 = wave, t1 = wave + NTHREADS / 64;
+    const int ntile = rtiles * ctiles;
+    const bool has0 = t0 < ntile, has1 = t1 < ntile;
+    const int rt0 = has0 ? t0 / ctiles : 0, ct0 = has0 ? t0 - rt0 * ctiles : 0;
+    const int rt1 = has1 ? t1 / ctiles : 0, ct1 = has1 ? t1 - rt1 * ctiles : 0;
+    const int astride = x.cplx ? 1 : 2;
+    T av0[BFast<T>::KS], av1[BFast<T>::KS];
+    {
+      const int row = rt0 * 16 + lr;
+      const bool rv = has0 && row < rows;
+      const int abase = x.cplx ? row * ld * 2 : (row >> 1) * ld * 2 + (row & 1);
+#pragma unroll
+      for (int c = 0; c < BFast<T>::KS; ++c) {
+        const int k = 4 * c + lk;
+        av0[c] = (rv && k < kdim) ? lld(&A[abase + k * astride]) : (T)0;
+      }
+    }
+    const bool same = rt1 == rt0;
+    if (!same) {
+      const int row = rt1 * 16 + lr;
+      const bool rv = has1 && row < rows;
+      const int abase = x.cplx ? row * ld * 2 : (row >> 1) * ld * 2 + (row & 1);
+#pragma unroll
+      for (int c = 0; c < BFast<T>::KS; ++c) {
+        const int k = 4 * c + lk;
+        av1[c] = (rv && k < kdim) ? lld(&A[abase + k * astride]) : (T)0;
+      }
+    }
+    typename M::acc_t acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < BFast<T>::KS; ++c)
+      if (4 * c < kpad) {
+        acc0 = M::run(av0[c], br.b[0][c], acc0);
+        acc1 = M::run(same ? av0[c] : av1[c], br.b[1][c], acc1);
+      }
+    if (has0) mat_store_tile<T, T2>(Cm, acc0, rt0, ct0, rows, ncols, ld, x.cplx, lr, lk);
+    if (has1) mat_store_tile<T, T2>(Cm, acc1, rt1, ct1, rows, ncols, ld, x.cplx, lr, lk);
+    return;
+  }
+  constexpr int KC = KChunk<T>::value;
+  const T* __restrict__ B = (const T*)x.B;
   for (int t = wave; t < rtiles * ctiles; t += NTHREADS / 64) {
     const int rt = t / ctiles;
     const int ct = t - rt * ctiles;
     const int row = rt * 16 + lr;
     const bool rv = row < rows;
-    int abase, astride;
-    if (x.cplx) {
-      abase = row * ld * 2;
-      astride = 1;
-    } else {
-      abase = (row >> 1) * ld * 2 + (row & 1);
-      astride = 2;
-    }
+    const int abase = x.cplx ? row * ld * 2 : (row >> 1) * ld * 2 + (row & 1);
+    const int astride = x.cplx ? 1 : 2;
     typename M::acc_t acc = {0, 0, 0, 0};
     const T* bcol = B + ct * 16 + lr;
-    for (int kk = 0; kk < kpad; kk += 4) {
-      const int k = kk + lk;
-      const T a = (rv && k < kdim) ? A[abase + k * astride] : (T)0;
-      const T b = bcol[(int64_t)k * npad];
-      acc = M::run(a, b, acc);
-    }
+    for (int kk0 = 0; kk0 < kpad; kk0 += 4 * KC) {
+      T av[KC], bv[KC];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = rt * 16 + M::row(lk, j);
-      const int col = ct * 16 + lr;
-      if (r < rows && col < ncols) {
-        if (x.cplx)
-          Cm[r * ld * 2 + col] = acc[j];
-        else
-          Cm[((r >> 1) * ld + col) * 2 + (r & 1)] = acc[j];
+      for (int c = 0; c < KC; ++c) {
+        const int k = kk0 + 4 * c + lk;
+        bv[c] = (k < kpad) ? gld(&bcol[(int64_t)k * npad]) : (T)0;
+        av[c] = (rv && k < kdim) ? lld(&A[abase + k * astride]) : (T)0;
       }
+#pragma unroll
+      for (int c = 0; c < KC; ++c)
+        if (kk0 + 4 * c < kpad) acc = M::run(av[c], bv[c], acc);
     }
+    mat_store_tile<T, T2>(Cm, acc, rt, ct, rows, ncols, ld, x.cplx, lr, lk);
   }
 }
 
@@ -336,7 +497,6 @@ __device__ void setup_lanes(const PassDesc* __restrict__ d, LaneTab& lt, int64_t
     }
     lt.valid[l] = valid;
     lt.base[l] = o * n * inner + i;
-    // coordinates of the other axes
     int64_t rem = o;
     for (int b = axis - 1; b >= 0; --b) {
       lt.idx[l][b] = (int32_t)(rem % d->shape[b]);
@@ -351,61 +511,108 @@ __device__ void setup_lanes(const PassDesc* __restrict__ d, LaneTab& lt, int64_t
     double w = 1.0;
     if (d->has_norm) {
       for (int b = 0; b < d->ndim; ++b)
-        if (b != axis) w *= d->w_other[b][lt.idx[l][b]];
+        if (b != axis) w *= gld(&d->w_other[b][lt.idx[l][b]]);
     }
     lt.w[l] = w;
   }
 }
 
-template <typename T, typename T2>
-__device__ __forceinline__ void apply_stage(const Stage& st, T2& z, int l, int k, const LaneTab& lt,
-                                            const PassDesc* __restrict__ d, double& a0, double& a1, double& a2,
-                                            double& a3) {
+struct Acc {
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  __device__ __forceinline__ void add(int which, double v) {
+    if (which == 0)
+      a0 += v;
+    else if (which == 1)
+      a1 += v;
+    else if (which == 2)
+      a2 += v;
+    else
+      a3 += v;
+  }
+};
+
+// Applies one element-wise stage to the EPT elements a thread holds. The op
+// loop is outermost and the element loop unrolled inside it, so every table
+// load of an op is in flight at once. Tables staged in LDS (`e.lds`) and
+// full-grid tables prefetched into registers (`e.pf`) are read from there.
+template <typename T, int EPT, int PFN, typename T2>
+__device__ __forceinline__ void stage_vec(const Stage& st, T2 (&z)[EPT], const int (&ql)[EPT], const int (&qk)[EPT],
+                                          int tid, int tot, const LaneTab& lt, const PassDesc* __restrict__ d,
+                                          Acc& acc, const unsigned char* lds_base, const T2 (&pf)[PFN > 0 ? PFN : 1][EPT]) {
   const int nops = st.n;
+  const int axis = d->axis;
+  const int64_t inner = d->inner;
   for (int i = 0; i < nops; ++i) {
     const ElemOp& e = st.op[i];
-    switch (e.kind) {
-      case EK_MUL1D: {
-        const int idx = (e.axis == d->axis) ? k : lt.idx[l][e.axis];
-        z = cmul(z, ((const T2*)e.table)[idx]);
-        break;
+    const int kind = e.kind;
+    if (kind == EK_MUL1D) {
+      const int ax = e.axis;
+      if (e.lds >= 0) {
+        const T2* tab = (const T2*)(lds_base + d->lds_arr_off[e.lds]);
+#pragma unroll
+        for (int q = 0; q < EPT; ++q)
+          if (tid + q * NTHREADS < tot) z[q] = cmul(z[q], lld(&tab[qk[q]]));
+      } else {
+        const T2* tab = (const T2*)e.table;
+#pragma unroll
+        for (int q = 0; q < EPT; ++q)
+          if (tid + q * NTHREADS < tot) {
+            const int idx = (ax == axis) ? qk[q] : lt.idx[ql[q]][ax];
+            z[q] = cmul(z[q], gld(&tab[idx]));
+          }
       }
-      case EK_MULND:
-        z = cmul(z, ((const T2*)e.table)[lt.base[l] + (int64_t)k * d->inner]);
-        break;
-      case EK_MULB:
-        z = cmul(z, ((const T2*)e.table)[0]);
-        break;
-      case EK_SCALE: {
-        const T s = (T)d->S[e.slot];
-        z.x *= s;
-        z.y *= s;
-        break;
+    } else if (kind == EK_MULND) {
+      if (PFN > 0 && e.pf >= 0) {
+#pragma unroll
+        for (int u = 0; u < (PFN > 0 ? PFN : 1); ++u)
+          if (u == e.pf) {
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) z[q] = cmul(z[q], pf[u][q]);
+          }
+      } else {
+        const T2* tab = (const T2*)e.table;
+#pragma unroll
+        for (int q = 0; q < EPT; ++q)
+          if (tid + q * NTHREADS < tot) z[q] = cmul(z[q], gld(&tab[lt.base[ql[q]] + (int64_t)qk[q] * inner]));
       }
-      case EK_NORM: {
-        const double v = ((double)z.x * (double)z.x + (double)z.y * (double)z.y) * (e.w_axis[k] * lt.w[l]);
-        switch (e.acc) {
-          case 0: a0 += v; break;
-          case 1: a1 += v; break;
-          case 2: a2 += v; break;
-          default: a3 += v; break;
-        }
-        break;
+    } else if (kind == EK_MULB) {
+      const T2 c = gld((const T2*)e.table);
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) z[q] = cmul(z[q], c);
+    } else if (kind == EK_SCALE) {
+      const T sc = (T)gld(&d->S[e.slot]);
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+        z[q].x *= sc;
+        z[q].y *= sc;
       }
-      default: break;
+    } else if (kind == EK_NORM) {
+      double sum = 0.0;
+      if (e.lds >= 0) {
+        const double* w = (const double*)(lds_base + d->lds_arr_off[e.lds]);
+#pragma unroll
+        for (int q = 0; q < EPT; ++q)
+          if (tid + q * NTHREADS < tot)
+            sum += ((double)z[q].x * (double)z[q].x + (double)z[q].y * (double)z[q].y) * (lld(&w[qk[q]]) * lt.w[ql[q]]);
+      } else {
+        const double* w = e.w_axis;
+#pragma unroll
+        for (int q = 0; q < EPT; ++q)
+          if (tid + q * NTHREADS < tot)
+            sum += ((double)z[q].x * (double)z[q].x + (double)z[q].y * (double)z[q].y) * (gld(&w[qk[q]]) * lt.w[ql[q]]);
+      }
+      acc.add(e.acc, sum);
     }
   }
 }
 
-__device__ void run_scalar_program(const PassDesc* __restrict__ d, const double* acc) {
-  double* S = d->S;
-  const int ns = d->nscalar;
+__device__ void run_program(const ScalarOp* sc, int ns, double* S, const double* acc, int nonfinite_slot) {
   for (int i = 0; i < ns; ++i) {
-    const ScalarOp& s = d->sc[i];
+    const ScalarOp& s = sc[i];
     switch (s.kind) {
       case SK_SET_ACC:
         S[s.a] = acc[s.b];
-        if (!isfinite(acc[s.b]) && d->nonfinite_slot >= 0) S[d->nonfinite_slot] = 1.0;
+        if (!isfinite(acc[s.b]) && nonfinite_slot >= 0) S[nonfinite_slot] = 1.0;
         break;
       case SK_COPY: S[s.a] = S[s.b]; break;
       case SK_LOSS: {
@@ -425,14 +632,66 @@ __device__ void run_scalar_program(const PassDesc* __restrict__ d, const double*
   }
 }
 
+__device__ void run_scalar_program(const PassDesc* __restrict__ d, const double* acc) {
+  run_program(d->sc, d->nscalar, d->S, acc, d->nonfinite_slot);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
 
-// Block reduction of the four accumulators, per-tile partials, and the
-// last-workgroup finalisation (fixed summation order over tiles).
-__device__ void finish_pass(const PassDesc* __restrict__ d, double a0, double a1, double a2, double a3, int tid) {
+// Sums per-tile partials in a fixed order (strided per thread, then wave and
+// block trees); every thread of the block gets the four totals.
+__device__ void sum_partials(const double* partials, int64_t nt, int tid, double* out, bool nontemporal) {
+  __shared__ double red2[NTHREADS / 64][MAX_ACC];
+  const int wave = tid >> 6, lane = tid & 63;
+  double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+  for (int64_t t = tid; t < nt; t += NTHREADS) {
+    const double* p = partials + t * MAX_ACC;
+    if (nontemporal) {
+      t0 += __builtin_nontemporal_load(p + 0);
+      t1 += __builtin_nontemporal_load(p + 1);
+      t2 += __builtin_nontemporal_load(p + 2);
+      t3 += __builtin_nontemporal_load(p + 3);
+    } else {
+      t0 += gld(p + 0);
+      t1 += gld(p + 1);
+      t2 += gld(p + 2);
+      t3 += gld(p + 3);
+    }
+  }
+  t0 = wave_sum(t0);
+  t1 = wave_sum(t1);
+  t2 = wave_sum(t2);
+  t3 = wave_sum(t3);
+  if (lane == 0) {
+    red2[wave][0] = t0;
+    red2[wave][1] = t1;
+    red2[wave][2] = t2;
+    red2[wave][3] = t3;
+  }
+  __syncthreads();
+  for (int j = 0; j < MAX_ACC; ++j) {
+    double s = 0.0;
+    for (int w = 0; w < NTHREADS / 64; ++w) s += red2[w][j];
+    out[j] = s;
+  }
+}
+
+// Workgroup 0 runs the scalar program the previous pass deferred (its norm
+// partials are visible here because a kernel boundary separates the passes).
+__device__ void run_inherited(const PassDesc* __restrict__ d, int tid) {
+  if (d->prev_nscalar == 0 || blockIdx.x != 0) return;
+  double acc[MAX_ACC];
+  sum_partials(d->prev_partials, d->prev_ntiles, tid, acc, false);
+  if (tid == 0) run_program(d->prev_sc, d->prev_nscalar, d->S, acc, d->nonfinite_slot);
+}
+
+// Block reduction of the four accumulators into per-tile partials. FIN_DEFERRED
+// stops there (the next pass finishes the program); FIN_IMMEDIATE lets the last
+// workgroup to arrive (release/acquire on a counter) sum them and run the program.
+__device__ void finish_pass(const PassDesc* __restrict__ d, const Acc& in, int tid) {
   __shared__ double red[NTHREADS / 64][MAX_ACC];
   __shared__ int is_last;
   const int64_t tile = blockIdx.x;
@@ -444,10 +703,7 @@ __device__ void finish_pass(const PassDesc* __restrict__ d, double a0, double a1
     return;
   }
   const int wave = tid >> 6, lane = tid & 63;
-  a0 = wave_sum(a0);
-  a1 = wave_sum(a1);
-  a2 = wave_sum(a2);
-  a3 = wave_sum(a3);
+  double a0 = wave_sum(in.a0), a1 = wave_sum(in.a1), a2 = wave_sum(in.a2), a3 = wave_sum(in.a3);
   if (lane == 0) {
     red[wave][0] = a0;
     red[wave][1] = a1;
@@ -459,8 +715,11 @@ __device__ void finish_pass(const PassDesc* __restrict__ d, double a0, double a1
     for (int j = 0; j < MAX_ACC; ++j) {
       double s = 0.0;
       for (int w = 0; w < NTHREADS / 64; ++w) s += red[w][j];
-      d->partials[tile * MAX_ACC + j] = s;
+      gst(&d->partials[tile * MAX_ACC + j], s);
     }
+  }
+  if (d->fin_mode != FIN_IMMEDIATE) return;
+  if (tid == 0) {
     __threadfence();
     const unsigned prev = atomicAdd(d->counter, 1u);
     is_last = (prev == (unsigned)(gridDim.x - 1));
@@ -468,35 +727,9 @@ __device__ void finish_pass(const PassDesc* __restrict__ d, double a0, double a1
   __syncthreads();
   if (!is_last) return;
   __threadfence();
-  // fixed-order sum over tiles: strided per thread, then wave and block trees
-  const int64_t nt = d->ntiles;
-  double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
-  for (int64_t t = tid; t < nt; t += NTHREADS) {
-    const double* p = d->partials + t * MAX_ACC;
-    t0 += __builtin_nontemporal_load(p + 0);
-    t1 += __builtin_nontemporal_load(p + 1);
-    t2 += __builtin_nontemporal_load(p + 2);
-    t3 += __builtin_nontemporal_load(p + 3);
-  }
-  t0 = wave_sum(t0);
-  t1 = wave_sum(t1);
-  t2 = wave_sum(t2);
-  t3 = wave_sum(t3);
-  __syncthreads();
-  if (lane == 0) {
-    red[wave][0] = t0;
-    red[wave][1] = t1;
-    red[wave][2] = t2;
-    red[wave][3] = t3;
-  }
-  __syncthreads();
+  double acc[MAX_ACC];
+  sum_partials(d->partials, d->ntiles, tid, acc, true);
   if (tid == 0) {
-    double acc[MAX_ACC];
-    for (int j = 0; j < MAX_ACC; ++j) {
-      double s = 0.0;
-      for (int w = 0; w < NTHREADS / 64; ++w) s += red[w][j];
-      acc[j] = s;
-    }
     run_scalar_program(d, acc);
     *d->counter = 0u;
     __threadfence();
@@ -504,63 +737,131 @@ __device__ void finish_pass(const PassDesc* __restrict__ d, double a0, double a1
 }
 
 // ---------------------------------------------------------------- the pass kernels
-template <typename T>
+// Element-wise pass (no transform): a flat, coalesced sweep; each workgroup
+// owns NTHREADS * EPT consecutive elements.
+template <typename T, int EPT>
 __global__ __launch_bounds__(NTHREADS) void elem_pass_kernel(const PassDesc* __restrict__ d) {
   using T2 = typename CT<T>::T2;
-  __shared__ LaneTab lt;
   const int tid = threadIdx.x;
-  const int64_t tile = blockIdx.x;
-  setup_lanes(d, lt, tile, tid);
-  __syncthreads();
+  run_inherited(d, tid);
   T2* psi = (T2*)d->psi;
-  const int tl = d->tl;
-  const int n = (int)d->n;
-  const int64_t inner = d->inner;
-  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-  const int tot = tl * n;
-  for (int f = tid; f < tot; f += NTHREADS) {
-    int l, k;
-    if (inner > 1) {
-      l = f % tl;
-      k = f / tl;
+  const int nd = d->ndim;
+  int64_t stride[MAXD];
+  int64_t size = 1;
+  for (int a = MAXD - 1; a >= 0; --a) {
+    stride[a] = size;
+    if (a < nd) size *= d->shape[a];
+  }
+  const int64_t base = (int64_t)blockIdx.x * (NTHREADS * EPT);
+  T2 z[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int64_t g = base + q * NTHREADS + tid;
+    if (g < size) {
+      z[q] = gld(&psi[g]);
     } else {
-      l = f / n;
-      k = f - l * n;
+      z[q].x = 0;
+      z[q].y = 0;
     }
-    if (!lt.valid[l]) continue;
-    const int64_t addr = lt.base[l] + (int64_t)k * inner;
-    T2 z = psi[addr];
-    apply_stage<T>(d->st[0], z, l, k, lt, d, a0, a1, a2, a3);
-    psi[addr] = z;
   }
-  finish_pass(d, a0, a1, a2, a3, tid);
+  Acc acc;
+  const Stage& st = d->st[0];
+  const int last = nd - 1;
+  for (int i = 0; i < st.n; ++i) {
+    const ElemOp& e = st.op[i];
+    const int kind = e.kind;
+    if (kind == EK_MUL1D) {
+      const T2* tab = (const T2*)e.table;
+      const int64_t sa = stride[e.axis], na = d->shape[e.axis];
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+        const int64_t g = base + q * NTHREADS + tid;
+        if (g < size) z[q] = cmul(z[q], gld(&tab[(g / sa) % na]));
+      }
+    } else if (kind == EK_MULND) {
+      const T2* tab = (const T2*)e.table;
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+        const int64_t g = base + q * NTHREADS + tid;
+        if (g < size) z[q] = cmul(z[q], gld(&tab[g]));
+      }
+    } else if (kind == EK_MULB) {
+      const T2 c = gld((const T2*)e.table);
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) z[q] = cmul(z[q], c);
+    } else if (kind == EK_SCALE) {
+      const T sc = (T)gld(&d->S[e.slot]);
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+        z[q].x *= sc;
+        z[q].y *= sc;
+      }
+    } else if (kind == EK_NORM) {
+      double sum = 0.0;
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+        const int64_t g = base + q * NTHREADS + tid;
+        if (g < size) {
+          double w = e.w_axis[g % d->shape[last]];
+          for (int a = 0; a < last; ++a) w *= d->w_other[a][(g / stride[a]) % d->shape[a]];
+          sum += ((double)z[q].x * (double)z[q].x + (double)z[q].y * (double)z[q].y) * w;
+        }
+      }
+      acc.add(e.acc, sum);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int64_t g = base + q * NTHREADS + tid;
+    if (g < size) gst(&psi[g], z[q]);
+  }
+  finish_pass(d, acc, tid);
 }
 
-template <typename T, int KIND, typename T2>
-__device__ __forceinline__ void run_xform(const XformDesc& x, T2*& cur, T2*& oth, int tl, int n, int ld, int tid) {
-  if constexpr (KIND == XF_FFT) {
-    fft_lds<T>(cur, tl, n, ld, x, tid);
-  } else {
-    mat_lds<T>(cur, oth, tl, ld, x, tid);
-    __syncthreads();
-    T2* t = cur;
-    cur = oth;
-    oth = t;
+// LDS position of element k of a lane: padded in FFT passes, plain in matrix passes
+template <int KIND>
+__device__ __forceinline__ int lidx(int k) {
+  if constexpr (KIND == XF_FFT)
+    return fpad(k);
+  else
+    return k;
+}
+
+template <int EPT>
+__device__ __forceinline__ void map_global(int (&ql)[EPT], int (&qk)[EPT], int tid, int tot, int tl, int n,
+                                           bool strided) {
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int f = tid + q * NTHREADS;
+    int l = 0, k = 0;
+    if (f < tot) {
+      if (strided) {
+        l = f % tl;
+        k = f / tl;
+      } else {
+        l = f / n;
+        k = f - l * n;
+      }
+    }
+    ql[q] = l;
+    qk[q] = k;
   }
 }
 
-// KIND = XF_FFT: every transform of the pass is an FFT; XF_MAT: every one is a
-// matrix product (passes never mix the two).
-template <typename T, int KIND>
+// Lane-transform pass. KIND = XF_FFT: every transform is an FFT (twiddle table
+// staged in LDS); XF_MAT: every transform is a matrix product on MFMA (two LDS
+// tiles, ping-pong; B prefetched into registers on the fast path). EPT =
+// complex elements per thread of the tile. Every global read the pass needs
+// (psi, full-grid tables of later stages, 1-D tables and norm weights, GEMM B)
+// is issued at kernel start, so the memory latency is paid once.
+template <typename T, int KIND, int EPT>
 __global__ __launch_bounds__(NTHREADS) void xform_pass_kernel(const PassDesc* __restrict__ d) {
   using T2 = typename CT<T>::T2;
+  constexpr int PFN = EPT <= 8 ? MAX_PF : 0;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ LaneTab lt;
   const int tid = threadIdx.x;
   const int64_t tile = blockIdx.x;
-  setup_lanes(d, lt, tile, tid);
-  __syncthreads();
-  T2* psi = (T2*)d->psi;
   const int tl = d->tl;
   const int n = (int)d->n;
   const int ld = d->ld;
@@ -568,63 +869,114 @@ __global__ __launch_bounds__(NTHREADS) void xform_pass_kernel(const PassDesc* __
   const int tot = tl * n;
   T2* cur = (T2*)smem;
   T2* oth = cur + tl * ld;
-  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-  // load + stage 0
-  for (int f = tid; f < tot; f += NTHREADS) {
-    int l, k;
-    if (inner > 1) {
-      l = f % tl;
-      k = f / tl;
-    } else {
-      l = f / n;
-      k = f - l * n;
+  const T2* twl = (const T2*)(smem + d->lds_arr_off[0]);
+  // stage small arrays (twiddles, 1-D tables, weights) into LDS: all loads of
+  // a chunk are issued before any store, so the latency is paid once
+  {
+    const int na = d->n_lds_arr;
+    int maxw = 0;
+    for (int a = 0; a < na; ++a) maxw = max(maxw, d->lds_arr_bytes[a] >> 3);
+    constexpr int CW = 8;
+    for (int c0 = 0; c0 < maxw; c0 += CW * NTHREADS) {
+      uint2 tmp[MAX_LDS_ARR][CW];
+#pragma unroll
+      for (int a = 0; a < MAX_LDS_ARR; ++a)
+        if (a < na) {
+          const uint2* src = (const uint2*)d->lds_arr_src[a];
+          const int nw = d->lds_arr_bytes[a] >> 3;
+#pragma unroll
+          for (int q = 0; q < CW; ++q) {
+            const int i = c0 + q * NTHREADS + tid;
+            if (i < nw) tmp[a][q] = gld(&src[i]);
+          }
+        }
+#pragma unroll
+      for (int a = 0; a < MAX_LDS_ARR; ++a)
+        if (a < na) {
+          uint2* dst = (uint2*)(smem + d->lds_arr_off[a]);
+          const int nw = d->lds_arr_bytes[a] >> 3;
+#pragma unroll
+          for (int q = 0; q < CW; ++q) {
+            const int i = c0 + q * NTHREADS + tid;
+            if (i < nw) lst(&dst[i], tmp[a][q]);
+          }
+        }
     }
-    T2 z;
-    if (lt.valid[l]) {
-      z = psi[lt.base[l] + (int64_t)k * inner];
-      apply_stage<T>(d->st[0], z, l, k, lt, d, a0, a1, a2, a3);
-    } else {
-      z.x = 0;
-      z.y = 0;
-    }
-    cur[l * ld + k] = z;
   }
+  BRegs<T> br;
+  if constexpr (KIND == XF_MAT) {
+    if (mat_fast<T>(d->xf[0], tl)) mat_prefetch<T>(d->xf[0], tl, tid, br);
+  }
+  setup_lanes(d, lt, tile, tid);
   __syncthreads();
-  run_xform<T, KIND>(d->xf[0], cur, oth, tl, n, ld, tid);
-  const int nx = d->nx;
-  if (nx == 2) {
-    if (d->st[1].n > 0) {
-      for (int f = tid; f < tot; f += NTHREADS) {
-        const int l = f / n, k = f - l * n;
-        if (!lt.valid[l]) continue;
-        T2 z = cur[l * ld + k];
-        apply_stage<T>(d->st[1], z, l, k, lt, d, a0, a1, a2, a3);
-        cur[l * ld + k] = z;
+  T2* psi = (T2*)d->psi;
+  const bool strided = inner > 1;
+  int ql[EPT], qk[EPT];
+  map_global<EPT>(ql, qk, tid, tot, tl, n, strided);
+  T2 z[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    z[q].x = 0;
+    z[q].y = 0;
+    if (tid + q * NTHREADS < tot && lt.valid[ql[q]]) z[q] = gld(&psi[lt.base[ql[q]] + (int64_t)qk[q] * inner]);
+  }
+  // full-grid tables of the later stages, into registers
+  T2 pf[PFN > 0 ? PFN : 1][EPT];
+  if constexpr (PFN > 0) {
+#pragma unroll
+    for (int u = 0; u < PFN; ++u)
+      if (u < d->n_pf) {
+        const T2* tab = (const T2*)d->pf_src[u];
+#pragma unroll
+        for (int q = 0; q < EPT; ++q)
+          if (tid + q * NTHREADS < tot) pf[u][q] = gld(&tab[lt.base[ql[q]] + (int64_t)qk[q] * inner]);
       }
+  }
+  run_inherited(d, tid);
+  Acc acc;
+  stage_vec<T, EPT, PFN>(d->st[0], z, ql, qk, tid, tot, lt, d, acc, smem, pf);
+#pragma unroll
+  for (int q = 0; q < EPT; ++q)
+    if (tid + q * NTHREADS < tot) lst(&cur[ql[q] * ld + lidx<KIND>(qk[q])], z[q]);
+  __syncthreads();
+  const int nx = d->nx;
+  for (int xi = 0; xi < nx; ++xi) {
+    const XformDesc& x = d->xf[xi];
+    if (xi == 1 && d->st[1].n > 0) {
+#pragma unroll
+      for (int q = 0; q < EPT; ++q)
+        if (tid + q * NTHREADS < tot) z[q] = lld(&cur[ql[q] * ld + lidx<KIND>(qk[q])]);
+      stage_vec<T, EPT, PFN>(d->st[1], z, ql, qk, tid, tot, lt, d, acc, smem, pf);
+#pragma unroll
+      for (int q = 0; q < EPT; ++q)
+        if (tid + q * NTHREADS < tot) lst(&cur[ql[q] * ld + lidx<KIND>(qk[q])], z[q]);
       __syncthreads();
     }
-    run_xform<T, KIND>(d->xf[1], cur, oth, tl, n, ld, tid);
-  }
-  // stage nx + store
-  const Stage& last = d->st[nx];
-  for (int f = tid; f < tot; f += NTHREADS) {
-    int l, k;
-    if (inner > 1) {
-      l = f % tl;
-      k = f / tl;
+    if constexpr (KIND == XF_FFT) {
+      fft_lds<EPT>(cur, tl, n, ld, twl, (T)x.scale, x.inverse != 0, x.nrad, x.rad, tid);
     } else {
-      l = f / n;
-      k = f - l * n;
+      const bool fx = mat_fast<T>(x, tl);
+      if (xi == 1 && fx) mat_prefetch<T>(x, tl, tid, br);
+      mat_lds<T>(cur, oth, tl, ld, x, tid, br, fx);
+      __syncthreads();
+      T2* t = cur;
+      cur = oth;
+      oth = t;
     }
-    if (!lt.valid[l]) continue;
-    T2 z = cur[l * ld + k];
-    apply_stage<T>(last, z, l, k, lt, d, a0, a1, a2, a3);
-    psi[lt.base[l] + (int64_t)k * inner] = z;
   }
-  finish_pass(d, a0, a1, a2, a3, tid);
+#pragma unroll
+  for (int q = 0; q < EPT; ++q)
+    if (tid + q * NTHREADS < tot) z[q] = lld(&cur[ql[q] * ld + lidx<KIND>(qk[q])]);
+  stage_vec<T, EPT, PFN>(d->st[nx], z, ql, qk, tid, tot, lt, d, acc, smem, pf);
+#pragma unroll
+  for (int q = 0; q < EPT; ++q)
+    if (tid + q * NTHREADS < tot && lt.valid[ql[q]]) gst(&psi[lt.base[ql[q]] + (int64_t)qk[q] * inner], z[q]);
+  finish_pass(d, acc, tid);
 }
 
-__global__ void scalar_kernel(const PassDesc* __restrict__ d) {
+__global__ __launch_bounds__(NTHREADS) void scalar_kernel(const PassDesc* __restrict__ d) {
+  run_inherited(d, threadIdx.x);
+  __syncthreads();
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     double acc[MAX_ACC] = {0, 0, 0, 0};
     run_scalar_program(d, acc);
@@ -799,53 +1151,68 @@ __global__ void gauss_legendre_kernel(int64_t n, double* x, double* w) {
 
 // ---------------------------------------------------------------- host launchers
 static size_t xform_lds_bytes(int dtype, const PassDesc& h) {
-  const size_t csz = dtype ? 16 : 8;
-  size_t bufs = 1;
-  for (int i = 0; i < h.nx; ++i)
-    if (h.xf[i].type == XF_MAT) bufs = 2;
-  return bufs * (size_t)h.tl * (size_t)h.ld * csz;
+  (void)dtype;
+  return (size_t)h.lds_bytes;
 }
+
+template <typename T, int KIND>
+static void launch_xform(const PassDesc* d_desc, int ept, dim3 grid, size_t lds, hipStream_t s) {
+  switch (ept) {
+    case 4: xform_pass_kernel<T, KIND, 4><<<grid, NTHREADS, lds, s>>>(d_desc); break;
+    case 8: xform_pass_kernel<T, KIND, 8><<<grid, NTHREADS, lds, s>>>(d_desc); break;
+    default: xform_pass_kernel<T, KIND, 16><<<grid, NTHREADS, lds, s>>>(d_desc); break;
+  }
+}
+
+template <typename T, int KIND>
+static void set_lds_attr() {
+  (void)hipFuncSetAttribute((const void*)xform_pass_kernel<T, KIND, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)SOD_MAX_DYN_LDS);
+  (void)hipFuncSetAttribute((const void*)xform_pass_kernel<T, KIND, 8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)SOD_MAX_DYN_LDS);
+  (void)hipFuncSetAttribute((const void*)xform_pass_kernel<T, KIND, 16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)SOD_MAX_DYN_LDS);
+}
+
+int sod_pass_lds_bytes(int dtype, const PassDesc& h) { return (int)xform_lds_bytes(dtype, h); }
 
 int sod_launch_pass(int dtype, const PassDesc* d_desc, const PassDesc& h, hipStream_t s) {
   const dim3 grid((unsigned)h.ntiles), block(NTHREADS);
   if (h.nx == 0) {
     if (dtype)
-      elem_pass_kernel<double><<<grid, block, 0, s>>>(d_desc);
+      elem_pass_kernel<double, SOD_ELEM_EPT><<<grid, block, 0, s>>>(d_desc);
     else
-      elem_pass_kernel<float><<<grid, block, 0, s>>>(d_desc);
+      elem_pass_kernel<float, SOD_ELEM_EPT><<<grid, block, 0, s>>>(d_desc);
   } else {
     const size_t lds = xform_lds_bytes(dtype, h);
     static bool attr_set = false;
     if (!attr_set) {
-      (void)hipFuncSetAttribute((const void*)xform_pass_kernel<double, XF_FFT>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)SOD_MAX_DYN_LDS);
-      (void)hipFuncSetAttribute((const void*)xform_pass_kernel<float, XF_FFT>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)SOD_MAX_DYN_LDS);
-      (void)hipFuncSetAttribute((const void*)xform_pass_kernel<double, XF_MAT>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)SOD_MAX_DYN_LDS);
-      (void)hipFuncSetAttribute((const void*)xform_pass_kernel<float, XF_MAT>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)SOD_MAX_DYN_LDS);
+      set_lds_attr<float, XF_FFT>();
+      set_lds_attr<float, XF_MAT>();
+      set_lds_attr<double, XF_FFT>();
+      set_lds_attr<double, XF_MAT>();
       attr_set = true;
     }
     if (lds > SOD_MAX_DYN_LDS) return (int)hipErrorInvalidValue;
+    if ((int64_t)h.tl * h.n > (int64_t)h.ept * NTHREADS) return (int)hipErrorInvalidValue;
     const bool mat = h.xf[0].type == XF_MAT;
     if (dtype) {
       if (mat)
-        xform_pass_kernel<double, XF_MAT><<<grid, block, lds, s>>>(d_desc);
+        launch_xform<double, XF_MAT>(d_desc, h.ept, grid, lds, s);
       else
-        xform_pass_kernel<double, XF_FFT><<<grid, block, lds, s>>>(d_desc);
+        launch_xform<double, XF_FFT>(d_desc, h.ept, grid, lds, s);
     } else {
       if (mat)
-        xform_pass_kernel<float, XF_MAT><<<grid, block, lds, s>>>(d_desc);
+        launch_xform<float, XF_MAT>(d_desc, h.ept, grid, lds, s);
       else
-        xform_pass_kernel<float, XF_FFT><<<grid, block, lds, s>>>(d_desc);
+        launch_xform<float, XF_FFT>(d_desc, h.ept, grid, lds, s);
     }
   }
   return (int)hipGetLastError();
 }
 
 int sod_launch_scalar(const PassDesc* d_desc, hipStream_t s) {
-  scalar_kernel<<<1, 64, 0, s>>>(d_desc);
+  scalar_kernel<<<1, NTHREADS, 0, s>>>(d_desc);
   return (int)hipGetLastError();
 }
 

@@ -512,19 +512,38 @@ struct Propagation {
   ~Propagation();
   int dtype() const { return wf ? wf->dtype : SO_COMPLEX_F32; }
 
+  // A pass whose norm bookkeeping is deferred: the next launched pass (or a
+  // scalar kernel) runs its scalar program in workgroup 0.
+  CompiledPass* pending_prog = nullptr;
+  std::string last_closed_key;
+
   // ----------------------------------------------------------- action emission
-  void close_open() {
+  // `immediate`: the next op reads a slot this pass's program writes, so the
+  // program must complete inside this pass (last-workgroup finalisation).
+  void close_open(bool immediate = false) {
     if (open.empty()) {
       open = PassB();
       return;
     }
+    // a pass that repeats back to back (one pass per step) finishes its own
+    // bookkeeping: deferring it would chain every launch to the previous one
+    std::string ck = pass_key(open);
+    if (ck == last_closed_key) immediate = true;
+    last_closed_key.swap(ck);
     Launch L;
-    L.pass = compile(open);
+    L.pass = compile(open, immediate);
     emitted.push_back(L);
     open = PassB();
   }
+  // launches a scalar kernel for a still-deferred program
+  void finish_pending() {
+    if (!pending_prog) return;
+    PassB empty;
+    CompiledPass* cp = compile(empty, false);
+    launch_pass(cp);
+  }
   void emit_elem(const ElemA& e) {
-    if (e.kind == sod::EK_SCALE && open.writes(e.slot)) close_open();
+    if (e.kind == sod::EK_SCALE && open.writes(e.slot)) close_open(true);
     if ((int)open.st[open.stage].size() >= sod::MAX_ELEM) close_open();
     open.st[open.stage].push_back(e);
   }
@@ -762,8 +781,9 @@ struct Propagation {
   }
 
   // ----------------------------------------------------------- compile
-  CompiledPass* compile(const PassB& b);
+  CompiledPass* compile(const PassB& b, bool immediate);
   void launch(const Launch& L);
+  void launch_pass(CompiledPass* cp);
   void run_saver(const SaveAct& s);
   void flush();
   void run_steps(int64_t n);
@@ -818,35 +838,68 @@ static void* gemm_operand(so_op* op, int which, int dtype, int* cplx_out) {
   return d;
 }
 
+// Twiddles grouped per Stockham pass, in the pass order of radix_plan(n): for
+// pass p (Ls = product of the earlier radices, radix R) the entries
+// W_n^(r k n/(Ls R)), r = 1..R-1, k = 0..Ls-1, stored at (r-1) Ls + k. The
+// groups add up to n - 1 entries. Values are computed in f64 and rounded once.
 static const void* fft_twiddles(so_op* op, int dtype) {
   if (op->twiddles.host.empty()) {
     const int64_t n = op->n;
     op->twiddles.ctx = op->ctx;
-    op->twiddles.host.resize((size_t)(2 * n));
-    for (int64_t e = 0; e < n; ++e) {
-      double c, s;
-      twiddle_exact(e, n, &c, &s);
-      op->twiddles.host[2 * e] = c;
-      op->twiddles.host[2 * e + 1] = -s;  // exp(-2 pi i e / n)
+    op->twiddles.host.assign((size_t)(2 * n), 0.0);
+    int64_t Ls = 1, off = 0;
+    for (int R : op->radices) {
+      const int64_t step = n / (Ls * R);
+      for (int64_t r = 1; r < R; ++r)
+        for (int64_t k = 0; k < Ls; ++k) {
+          double c, s;
+          twiddle_exact((r * k * step) % n, n, &c, &s);
+          const int64_t i = off + (r - 1) * Ls + k;
+          op->twiddles.host[2 * i] = c;
+          op->twiddles.host[2 * i + 1] = -s;  // exp(-2 pi i e / n)
+        }
+      off += (R - 1) * Ls;
+      Ls *= R;
     }
   }
   return op->twiddles.get(dtype);
 }
 
-CompiledPass* Propagation::compile(const PassB& b) {
+// LDS row pitch (complex elements) of a pass tile: FFT rows hold padded
+// indices (one pad per 32); matrix rows are spaced 8 banks apart (pitch = 4 mod 32).
+static int64_t lds_pitch(bool mat, int64_t n, int64_t tl) {
+  if (mat) return n + ((4 - n) % 32 + 32) % 32;
+  return n + (n >> 5) + (tl > 1 ? 1 : 0);
+}
+
+CompiledPass* Propagation::compile(const PassB& b, bool immediate) {
+  const bool scalar_only = b.nx == 0 && b.st[0].empty();
   std::string key = pass_key(b);
+  key_add(key, &pending_prog, sizeof(void*));
+  key.push_back(immediate ? 1 : 0);
   auto it = cache.find(key);
-  if (it != cache.end()) return it->second.get();
+  if (it != cache.end()) {
+    CompiledPass* c = it->second.get();
+    pending_prog = (c->h.has_norm && c->h.fin_mode == sod::FIN_DEFERRED) ? c : nullptr;
+    return c;
+  }
   auto cp = std::make_unique<CompiledPass>();
   sod::PassDesc& h = cp->h;
   memset(&h, 0, sizeof(h));
   const int dt = dtype();
   h.psi = wf->data;
+  if (pending_prog) {
+    const sod::PassDesc& ph = pending_prog->h;
+    h.prev_nscalar = ph.nscalar;
+    for (int i = 0; i < ph.nscalar; ++i) h.prev_sc[i] = ph.sc[i];
+    h.prev_partials = ph.partials;
+    h.prev_ntiles = ph.ntiles;
+  }
   h.ndim = wf->ndim;
   for (int a = 0; a < MAXD; ++a) h.shape[a] = a < wf->ndim ? wf->shape[a] : 1;
   h.S = ctx->arena;
   h.nonfinite_slot = ctx->nonfinite_slot;
-  if (b.nx == 0 && b.st[0].empty()) {
+  if (scalar_only) {
     // scalar bookkeeping only
     cp->scalar_only = true;
     h.nscalar = (int)b.sc.size();
@@ -892,34 +945,38 @@ CompiledPass* Propagation::compile(const PassB& b) {
     const int64_t lanes_avail = h.inner > 1 ? h.inner : h.outer;
     int tl;
     if (b.nx == 0) {
-      tl = (int)std::max<int64_t>(1, std::min<int64_t>(sod::MAX_TL, 4096 / std::max<int64_t>(n, 1)));
-      tl = (int)std::min<int64_t>(tl, lanes_avail);
+      // flat element-wise sweep
+      tl = 1;
       h.ld = (int)n;
     } else {
-      const int bufs = has_mat ? 2 : 1;
-      int64_t cap = sod::MAX_TL;
-      cap = std::min<int64_t>(cap, lanes_avail);
-      // LDS and FFT register budgets
-      while (cap > 1 && (int64_t)bufs * cap * (n + 1) * (int64_t)cs > SOD_MAX_DYN_LDS) cap /= 2;
-      bool has_fft = false;
-      for (int i = 0; i < b.nx; ++i) has_fft |= b.xf[i].type == sod::XF_FFT;
-      if (has_fft)
-        while (cap > 1 && cap * n > (int64_t)sod::FFT_EPT * sod::NTHREADS) cap /= 2;
+      const bool mat = has_mat;
+      // FFT twiddles must sit in LDS; 1-D tables and norm weights are staged
+      // there too when they fit (checked when the descriptor is filled)
+      const int64_t tw_bytes = mat ? 0 : (n * (int64_t)cs + 15) / 16 * 16;
+      const int bufs = mat ? 2 : 1;
+      auto lds_for = [&](int64_t t) { return (int64_t)bufs * t * lds_pitch(mat, n, t) * (int64_t)cs + tw_bytes; };
+      int64_t cap = std::min<int64_t>(sod::MAX_TL, lanes_avail);
+      while (cap > 1 && (lds_for(cap) > SOD_PREF_DYN_LDS || cap * n > (int64_t)sod::FFT_EPT * sod::NTHREADS)) cap /= 2;
       tl = (int)std::max<int64_t>(cap, 1);
       // prefer enough workgroups to fill 256 CUs
       auto tiles_for = [&](int t) {
         return h.inner > 1 ? h.outer * ((h.inner + t - 1) / t) : (h.outer + t - 1) / t;
       };
-      const int min_tl = has_mat ? 8 : 1;
+      const int min_tl = mat ? 8 : 1;
       while (tl > min_tl && tiles_for(tl) < 512) tl /= 2;
-      if (has_fft && tl * n > (int64_t)sod::FFT_EPT * sod::NTHREADS)
-        fail(SO_ERR_UNSUPPORTED, "FFT lane length %lld exceeds the LDS tile limit", (long long)n);
-      h.ld = (int)(tl > 1 ? n + 1 : n);
-      if ((int64_t)bufs * tl * h.ld * (int64_t)cs > SOD_MAX_DYN_LDS)
+      if ((int64_t)tl * n > (int64_t)sod::FFT_EPT * sod::NTHREADS || lds_for(tl) > SOD_MAX_DYN_LDS)
         fail(SO_ERR_UNSUPPORTED, "lane length %lld exceeds the LDS tile limit for this dtype", (long long)n);
+      h.ld = (int)lds_pitch(mat, n, tl);
+      const int64_t tot = (int64_t)tl * n;
+      h.ept = tot <= 4 * sod::NTHREADS ? 4 : tot <= 8 * sod::NTHREADS ? 8 : 16;
     }
     h.tl = tl;
-    if (h.inner > 1) {
+    if (b.nx == 0) {
+      int64_t size = 1;
+      for (int a = 0; a < wf->ndim; ++a) size *= wf->shape[a];
+      h.tiles_per_outer = 1;
+      h.ntiles = (size + sod::NTHREADS * SOD_ELEM_EPT - 1) / (sod::NTHREADS * SOD_ELEM_EPT);
+    } else if (h.inner > 1) {
       h.tiles_per_outer = (h.inner + tl - 1) / tl;
       h.ntiles = h.outer * h.tiles_per_outer;
     } else {
@@ -933,6 +990,8 @@ CompiledPass* Propagation::compile(const PassB& b) {
       for (size_t i = 0; i < b.st[s].size(); ++i) {
         const ElemA& e = b.st[s][i];
         sod::ElemOp& o = h.st[s].op[i];
+        o.lds = -1;
+        o.pf = -1;
         o.kind = e.kind;
         o.axis = e.axis;
         o.table = e.table;
@@ -950,6 +1009,42 @@ CompiledPass* Propagation::compile(const PassB& b) {
         }
       }
     }
+    // LDS arrays and register prefetch
+    if (b.nx > 0) {
+      const int bufs = has_mat ? 2 : 1;
+      int64_t off = ((int64_t)bufs * h.tl * h.ld * (int64_t)cs + 15) / 16 * 16;
+      auto add_arr = [&](const void* src, int64_t bytes) -> int {
+        for (int i = 0; i < h.n_lds_arr; ++i)
+          if (h.lds_arr_src[i] == src) return i;
+        const int64_t budget = std::max<int64_t>(SOD_PREF_DYN_LDS, off + (h.n_lds_arr == 0 && !has_mat ? bytes : 0));
+        if (h.n_lds_arr >= sod::MAX_LDS_ARR || off + bytes > std::min<int64_t>(budget, SOD_MAX_DYN_LDS)) return -1;
+        const int i = h.n_lds_arr++;
+        h.lds_arr_src[i] = src;
+        h.lds_arr_bytes[i] = (int32_t)bytes;
+        h.lds_arr_off[i] = (int32_t)off;
+        off += (bytes + 15) / 16 * 16;
+        return i;
+      };
+      if (!has_mat) {
+        if (add_arr(h.xf[0].tw, n * (int64_t)cs) != 0) fail(SO_ERR_UNSUPPORTED, "FFT twiddles do not fit in LDS");
+      }
+      auto add_pf = [&](const void* src) -> int {
+        if (h.ept > 8) return -1;
+        for (int i = 0; i < h.n_pf; ++i)
+          if (h.pf_src[i] == src) return i;
+        if (h.n_pf >= sod::MAX_PF) return -1;
+        h.pf_src[h.n_pf] = src;
+        return h.n_pf++;
+      };
+      for (int st = 0; st < 3; ++st)
+        for (int i = 0; i < h.st[st].n; ++i) {
+          sod::ElemOp& o = h.st[st].op[i];
+          if (o.kind == sod::EK_MUL1D && o.axis == axis) o.lds = add_arr(o.table, n * (int64_t)cs);
+          if (o.kind == sod::EK_NORM) o.lds = add_arr(o.w_axis, n * 8);
+          if (o.kind == sod::EK_MULND) o.pf = add_pf(o.table);
+        }
+      h.lds_bytes = (int32_t)off;
+    }
     h.nacc = b.nacc;
     h.has_norm = b.nacc > 0 ? 1 : 0;
     h.nscalar = (int)b.sc.size();
@@ -960,10 +1055,12 @@ CompiledPass* Propagation::compile(const PassB& b) {
       HIPCHK(hipMemset(h.counter, 0, sizeof(unsigned int) * 4));
     }
   }
+  h.fin_mode = h.has_norm ? (immediate ? sod::FIN_IMMEDIATE : sod::FIN_DEFERRED) : sod::FIN_NONE;
   cp->d = (sod::PassDesc*)ctx->alloc(sizeof(sod::PassDesc));
   HIPCHK(hipMemcpy(cp->d, &h, sizeof(sod::PassDesc), hipMemcpyHostToDevice));
   CompiledPass* raw = cp.get();
   cache.emplace(std::move(key), std::move(cp));
+  pending_prog = (raw->h.has_norm && raw->h.fin_mode == sod::FIN_DEFERRED) ? raw : nullptr;
   return raw;
 }
 
@@ -1000,7 +1097,10 @@ void Propagation::launch(const Launch& L) {
     run_saver(L.save);
     return;
   }
-  CompiledPass* cp = L.pass;
+  launch_pass(L.pass);
+}
+
+void Propagation::launch_pass(CompiledPass* cp) {
   if (cp->scalar_only)
     HIPCHK((hipError_t)sod_launch_scalar(cp->d, ctx->stream));
   else
@@ -1008,15 +1108,16 @@ void Propagation::launch(const Launch& L) {
 }
 
 void Propagation::flush() {
-  if (!open.empty()) {
-    close_open();
-    for (auto& L : emitted) launch(L);
-    emitted.clear();
-  }
+  if (!open.empty()) close_open();
+  for (auto& L : emitted) launch(L);
+  emitted.clear();
+  finish_pending();
 }
 
 std::string Propagation::state_sig() const {
   std::string k = pass_key(open);
+  key_add(k, &pending_prog, sizeof(void*));
+  k += last_closed_key;
   k.push_back(wf->possible_norm_change ? 1 : 0);
   for (int a = 0; a < wf->ndim; ++a) {
     k += wf->grids[a].name;
@@ -1236,11 +1337,7 @@ struct ImmediatePass {
     (void)hipStreamSynchronize(p.ctx->stream);
     p.wf = nullptr;
   }
-  void run() {
-    p.close_open();
-    for (auto& L : p.emitted) p.launch(L);
-    p.emitted.clear();
-  }
+  void run() { p.flush(); }
 };
 
 static double wf_norm_value(so_wave_function* wf) {
@@ -2014,6 +2111,8 @@ static void prop_reset_compiled(Propagation& p) {
   p.cache.clear();
   p.open = PassB();
   p.emitted.clear();
+  p.pending_prog = nullptr;
+  p.last_closed_key.clear();
 }
 
 extern "C" int so_propagation_set_wave_function(so_propagation* pr, so_wave_function* src) {
