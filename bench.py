@@ -36,6 +36,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=20, help="steps of the CPU oracle baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=200)
+    ap.add_argument("--c3-steps", type=int, default=100, help="steps of the 256^3 Cartesian workload (0: skip)")
     return ap.parse_args()
 
 
@@ -54,6 +55,25 @@ def pass_record(t):
     if t["mat_flops"] > 0:
         rec["tflops"] = t["mat_flops"] / sec / 1e12
     return rec
+
+
+def cartesian_256(so, W, steps, kernel_iters):
+    """The 256^3 Cartesian workload (complex FP32): throughput and per-pass rates."""
+    import time
+
+    p, _ = W.cartesian_3d(so, n=256, steps=steps, dtype="f32")
+    p.steps(5)
+    so.synchronize()
+    t0 = time.perf_counter()
+    p.steps(steps)
+    so.synchronize()
+    wall = time.perf_counter() - t0
+    pts = 256 ** 3
+    passes = [pass_record(t) for t in p.time_passes(max(10, kernel_iters // 10))]
+    step_bytes = sum(r["alg_bytes"] for r in passes)
+    return {"value": pts * steps / wall, "unit": "grid-point-steps/s", "ms_per_step": wall / steps * 1e3,
+            "steps": steps, "alg_bytes_per_step": step_bytes,
+            "step_gbs": step_bytes / (wall / steps) / 1e9, "passes": passes}
 
 
 def main():
@@ -122,6 +142,10 @@ def main():
         cpu = {"value": pts * n_cpu / cw, "unit": "grid-point-steps/s", "cores": O.num_threads(), "kind": "port",
                "sample": f"f64 C oracle, same 2048x128 stack, {n_cpu} steps after 1 warm-up step ({cw:.1f} s)"}
 
+    extra = {}
+    if args.c3_steps > 0:
+        extra["cartesian_3d_256"] = cartesian_256(so, W, args.c3_steps, args.kernel_iters)
+
     line = {
         "metric": METRIC, "value": value, "unit": "grid-point-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
@@ -134,6 +158,7 @@ def main():
         "launches_per_step": stats["launches_per_step"],
         "rel_l2_vs_f64_gpu": l2,
         "passes": passes,
+        "extra_workloads": extra,
     }
     print(json.dumps(line))
 

@@ -22,7 +22,7 @@ constexpr int MAX_RADIX = 16;   // radix passes per FFT
 constexpr int MAX_TL = 64;      // lanes per workgroup tile
 constexpr int NTHREADS = 256;   // threads per workgroup (4 wave64)
 constexpr int MAX_LDS_ARR = 4;  // small per-pass arrays staged in LDS at kernel start
-constexpr int MAX_PF = 3;       // full-grid tables prefetched into registers at kernel start
+constexpr int MAX_PF = 2;       // full-grid tables prefetched into registers at kernel start
 constexpr int FFT_EPT = 16;     // max complex elements per thread of a transform tile (tl * n <= FFT_EPT * NTHREADS)
 #define SOD_MAX_DYN_LDS (152 * 1024)  // dynamic LDS bytes a pass may request per workgroup (gfx950: 160 KiB)
 #define SOD_PREF_DYN_LDS (64 * 1024)  // tiles are sized to this when a single lane fits, two workgroups per CU

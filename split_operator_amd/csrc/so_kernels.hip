@@ -137,20 +137,29 @@ __device__ __forceinline__ void dft4(T2* v, T sg) {
   v[3] = csub(t1, t3);
 }
 
+// c * x with c = c_hi + c_lo: the rounding of an irrational constant to T is
+// carried by c_lo, so repeated butterflies do not shrink or grow the norm
+template <typename T>
+__device__ __forceinline__ T cmul_const(T x, T c_hi, T c_lo) {
+  return fma(x, c_hi, x * c_lo);
+}
+
 template <typename T, typename T2>
 __device__ __forceinline__ void dft8(T2* v, T sg) {
   const T c = (T)0.70710678118654752440084436210485;
+  const T c_lo = (T)(0.70710678118654752440084436210485 - (double)c);
   T2 e[4] = {v[0], v[2], v[4], v[6]};
   T2 o[4] = {v[1], v[3], v[5], v[7]};
   dft4(e, sg);
   dft4(o, sg);
-  // o1 *= exp(-sg i pi/4), o3 *= exp(-sg 3 i pi/4)
-  const T cs = c * sg;
+  // o1 *= exp(-sg i pi/4) = c (1 - sg i), o3 *= exp(-sg 3 i pi/4) = c (-1 - sg i)
   T2 o1, o3;
-  o1.x = c * o[1].x + cs * o[1].y;
-  o1.y = c * o[1].y - cs * o[1].x;
-  o3.x = -c * o[3].x + cs * o[3].y;
-  o3.y = -c * o[3].y - cs * o[3].x;
+  const T a1 = o[1].x + sg * o[1].y, b1 = o[1].y - sg * o[1].x;
+  const T a3 = -o[3].x + sg * o[3].y, b3 = -o[3].y - sg * o[3].x;
+  o1.x = cmul_const(a1, c, c_lo);
+  o1.y = cmul_const(b1, c, c_lo);
+  o3.x = cmul_const(a3, c, c_lo);
+  o3.y = cmul_const(b3, c, c_lo);
   T2 o2 = rot90(o[2], sg);
   v[0] = cadd(e[0], o[0]);
   v[4] = csub(e[0], o[0]);
@@ -165,14 +174,15 @@ __device__ __forceinline__ void dft8(T2* v, T sg) {
 template <typename T, typename T2>
 __device__ __forceinline__ void dft3(T2* v, T sg) {
   const T s = (T)0.86602540378443864676372317075294;  // sin(2 pi / 3)
+  const T s_lo = (T)(0.86602540378443864676372317075294 - (double)s);
   T2 t = cadd(v[1], v[2]);
   T2 u = csub(v[1], v[2]);
   T2 m;
   m.x = v[0].x - (T)0.5 * t.x;
   m.y = v[0].y - (T)0.5 * t.y;
   T2 su;
-  su.x = s * u.x;
-  su.y = s * u.y;
+  su.x = cmul_const(u.x, s, s_lo);
+  su.y = cmul_const(u.y, s, s_lo);
   T2 r = rot90(su, sg);
   v[0] = cadd(v[0], t);
   v[1] = cadd(m, r);
@@ -199,8 +209,8 @@ __device__ __forceinline__ int fpad(int i) { return i + (i >> 5); }
 // wave read consecutive entries. All reads land in registers before the
 // barrier, so the pass is done in place.
 template <int R, int EPT, typename T, typename T2>
-__device__ __forceinline__ void radix_pass(T2* lds, int tl, int n, int ld, int Ls, const T2* tw, T scale, T sg,
-                                           bool last, int tid) {
+__device__ __forceinline__ void radix_pass(T2* lds, int tl, int n, int ld, int Ls, const T2* tw, T scale,
+                                           T scale_lo, T sg, bool last, int tid) {
   constexpr int MAXB = EPT / R > 0 ? EPT / R : 1;
   const int nb = n / R;
   const int total = tl * nb;
@@ -254,8 +264,10 @@ __device__ __forceinline__ void radix_pass(T2* lds, int tl, int n, int ld, int L
       for (int r = 0; r < R; ++r) {
         T2 z = v[q][r];
         if (last) {
-          z.x *= scale;
-          z.y *= scale;
+          // z * (scale + scale_lo): the rounding of 1/sqrt(n) to T does not
+          // accumulate into a norm drift over many transforms
+          z.x = fma(z.x, scale, z.x * scale_lo);
+          z.y = fma(z.y, scale, z.y * scale_lo);
         }
         lst(&dst[fpad(dsti[q] + r * Ls)], z);
       }
@@ -265,8 +277,10 @@ __device__ __forceinline__ void radix_pass(T2* lds, int tl, int n, int ld, int L
 }
 
 template <int EPT, typename T, typename T2>
-__device__ __forceinline__ void fft_lds(T2* lds, int tl, int n, int ld, const T2* tw, T scale, bool inverse, int nrad,
-                                        const int32_t* rad, int tid) {
+__device__ __forceinline__ void fft_lds(T2* lds, int tl, int n, int ld, const T2* tw, double scale_d, bool inverse,
+                                        int nrad, const int32_t* rad, int tid) {
+  const T scale = (T)scale_d;
+  const T scale_lo = (T)(scale_d - (double)scale);
   const T sg = inverse ? (T)-1 : (T)1;
   int Ls = 1;
   int off = 0;  // start of the current pass's twiddles
@@ -275,13 +289,13 @@ __device__ __forceinline__ void fft_lds(T2* lds, int tl, int n, int ld, const T2
     const bool last = (p == nrad - 1);
     const T2* twp = tw + off;
     if (R == 8)
-      radix_pass<8, EPT>(lds, tl, n, ld, Ls, twp, scale, sg, last, tid);
+      radix_pass<8, EPT>(lds, tl, n, ld, Ls, twp, scale, scale_lo, sg, last, tid);
     else if (R == 4)
-      radix_pass<4, EPT>(lds, tl, n, ld, Ls, twp, scale, sg, last, tid);
+      radix_pass<4, EPT>(lds, tl, n, ld, Ls, twp, scale, scale_lo, sg, last, tid);
     else if (R == 3)
-      radix_pass<3, EPT>(lds, tl, n, ld, Ls, twp, scale, sg, last, tid);
+      radix_pass<3, EPT>(lds, tl, n, ld, Ls, twp, scale, scale_lo, sg, last, tid);
     else
-      radix_pass<2, EPT>(lds, tl, n, ld, Ls, twp, scale, sg, last, tid);
+      radix_pass<2, EPT>(lds, tl, n, ld, Ls, twp, scale, scale_lo, sg, last, tid);
     off += (R - 1) * Ls;
     Ls *= R;
   }
@@ -876,7 +890,7 @@ __global__ __launch_bounds__(NTHREADS) void xform_pass_kernel(const PassDesc* __
     const int na = d->n_lds_arr;
     int maxw = 0;
     for (int a = 0; a < na; ++a) maxw = max(maxw, d->lds_arr_bytes[a] >> 3);
-    constexpr int CW = 8;
+    constexpr int CW = 4;
     for (int c0 = 0; c0 < maxw; c0 += CW * NTHREADS) {
       uint2 tmp[MAX_LDS_ARR][CW];
 #pragma unroll
@@ -953,7 +967,7 @@ __global__ __launch_bounds__(NTHREADS) void xform_pass_kernel(const PassDesc* __
       __syncthreads();
     }
     if constexpr (KIND == XF_FFT) {
-      fft_lds<EPT>(cur, tl, n, ld, twl, (T)x.scale, x.inverse != 0, x.nrad, x.rad, tid);
+      fft_lds<EPT, T>(cur, tl, n, ld, twl, x.scale, x.inverse != 0, x.nrad, x.rad, tid);
     } else {
       const bool fx = mat_fast<T>(x, tl);
       if (xi == 1 && fx) mat_prefetch<T>(x, tl, tid, br);

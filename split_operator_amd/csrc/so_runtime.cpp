@@ -226,6 +226,36 @@ struct LossCheckerH {
   double* d_hist = nullptr;
 };
 
+// Rounds a complex table entry to float. Entries of modulus 1 (exp(-i H dt) in
+// real time, masks of 1, twiddles) are rounded to the neighbouring float pair
+// (within one ulp of each component) whose modulus is closest to 1: the
+// rounding of a phase factor is the same at every step, so any bias of |z|
+// would accumulate into a norm drift over the propagation.
+static void round_unit_pair(double re, double im, float* fr, float* fi) {
+  float r = (float)re, i = (float)im;
+  const double m = re * re + im * im;
+  if (std::fabs(m - 1.0) < 1e-12) {
+    float br = r, bi = i;
+    double best = std::fabs((double)r * r + (double)i * i - 1.0);
+    for (int dr = -1; dr <= 1; ++dr)
+      for (int di = -1; di <= 1; ++di) {
+        float cr = r, ci = i;
+        if (dr) cr = std::nextafter(r, dr > 0 ? 2.0f : -2.0f);
+        if (di) ci = std::nextafter(i, di > 0 ? 2.0f : -2.0f);
+        const double e = std::fabs((double)cr * cr + (double)ci * ci - 1.0);
+        if (e < best) {
+          best = e;
+          br = cr;
+          bi = ci;
+        }
+      }
+    r = br;
+    i = bi;
+  }
+  *fr = r;
+  *fi = i;
+}
+
 // a complex table uploaded lazily in the storage dtype
 struct DevTable {
   std::vector<double> host;  // interleaved complex f64
@@ -245,7 +275,8 @@ struct DevTable {
         HIPCHK(hipMemcpy(dev[1], host.data(), sizeof(double) * n2, hipMemcpyHostToDevice));
       } else {
         std::vector<float> f(n2);
-        for (size_t i = 0; i < n2; ++i) f[i] = (float)host[i];
+        for (size_t i = 0; i + 1 < n2; i += 2) round_unit_pair(host[i], host[i + 1], &f[i], &f[i + 1]);
+        if (n2 % 2) f[n2 - 1] = (float)host[n2 - 1];
         dev[0] = ctx->alloc(sizeof(float) * std::max<size_t>(n2, 2));
         HIPCHK(hipMemcpy(dev[0], f.data(), sizeof(float) * n2, hipMemcpyHostToDevice));
       }
@@ -963,7 +994,10 @@ CompiledPass* Propagation::compile(const PassB& b, bool immediate) {
         return h.inner > 1 ? h.outer * ((h.inner + t - 1) / t) : (h.outer + t - 1) / t;
       };
       const int min_tl = mat ? 8 : 1;
-      while (tl > min_tl && tiles_for(tl) < 512) tl /= 2;
+      // tiles of at most 2048 elements (8 per thread) keep two or more
+      // workgroups resident per CU, so one tile's memory phase overlaps
+      // another's transform; halve further while fewer than 512 tiles
+      while (tl > min_tl && (tiles_for(tl) < 512 || (int64_t)tl * n > 2048)) tl /= 2;
       if ((int64_t)tl * n > (int64_t)sod::FFT_EPT * sod::NTHREADS || lds_for(tl) > SOD_MAX_DYN_LDS)
         fail(SO_ERR_UNSUPPORTED, "lane length %lld exceeds the LDS tile limit for this dtype", (long long)n);
       h.ld = (int)lds_pitch(mat, n, tl);

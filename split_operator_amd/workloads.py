@@ -107,3 +107,47 @@ def radial_angular(B, n_r=2048, n_theta=128, r_min=0.5, r_max=50.0, dt=0.002, st
     p.add_propagator(B.one_dim_propagator(B.exp_propagator(B.kinetic_hamiltonian(radial, 1.0),
                                                            B.select_step(True, dt, im_time)), 0))
     return p, (radial, polar), (c6, c12, a)
+
+
+def cartesian_3d_params(seed, bumps=8, box=15.0):
+    """Seeded bumps (height U[-0.5, 0.5], width U[0.5, 2], centre U[-box, box]^3) and k in U[-1, 1]^3."""
+    rng = SplitMix64(seed)
+    hs, ws, cs = [], [], []
+    for _ in range(bumps):
+        hs.append(rng.uniform(-0.5, 0.5))
+        ws.append(rng.uniform(0.5, 2.0))
+        cs.append([rng.uniform(-box, box) for _ in range(3)])
+    k = [rng.uniform(-1.0, 1.0) for _ in range(3)]
+    return np.array(hs), np.array(ws), np.array(cs), np.array(k)
+
+
+def cartesian_3d(B, n=256, box=15.0, dt=0.005, steps=500, seed=3000, dtype=None, omega=(1.0, 1.3, 1.7),
+                 sigma=1.5, im_time=False):
+    """3-D Cartesian propagation on [-15, 15]^3 (n^3 points): V = sum_a omega_a^2 x_a^2 / 2 plus 8 seeded
+    Gaussian bumps h exp(-|x - c|^2 / (2 w^2)); psi0 = Gaussian (sigma 1.5) with seeded momentum; m = 1.
+    Stack [V/2, FFT_x, FFT_y, FFT_z, T_x, T_y, T_z] (the crate's per-axis FFT transformations)."""
+    hs, ws, cs, k = cartesian_3d_params(seed, box=box)
+    grids = [B.Grid.new_linear_continuos(name, -box, box, n, a) for a, name in enumerate("xyz")]
+    x = grids[0].nodes
+    g1 = [np.array([B.gaussian_distribution(xi, 0.0, sigma, k[a]) for xi in x]) for a in range(3)]
+    psi0 = g1[0][:, None, None] * g1[1][None, :, None] * g1[2][None, None, :]
+    wf = B.WaveFunction(psi0, grids) if dtype is None else B.WaveFunction(psi0, grids, dtype=dtype)
+    wf.normalize(1.0)
+    del psi0
+    p = B.Propagation()
+    p.set_wave_function(wf)
+    p.set_time_grid(dt, steps, im_time)
+    X = x[:, None, None]
+    Y = x[None, :, None]
+    Z = x[None, None, :]
+    V = 0.5 * (omega[0] ** 2 * X * X + omega[1] ** 2 * Y * Y + omega[2] ** 2 * Z * Z)
+    for h, w, c in zip(hs, ws, cs):
+        V = V + h * np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) / (2.0 * w * w))
+    p.add_propagator(B.n_dim_propagator(B.exp_propagator(V, B.select_step(False, dt, im_time))))
+    del V
+    for a in range(3):
+        p.add_transformation(B.fft_transformation(grids[a], "k" + "xyz"[a]))
+    for a in range(3):
+        p.add_propagator(B.one_dim_propagator(B.exp_propagator(B.kinetic_hamiltonian(grids[a], 1.0),
+                                                               B.select_step(True, dt, im_time)), a))
+    return p, grids

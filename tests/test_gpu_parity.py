@@ -280,3 +280,26 @@ def test_savers_and_loss_saver(tmp_path, monkeypatch):
     assert (tmp_path / "wf2d_x_grid.npy").exists() and (tmp_path / "border.dat").exists()
     saved = np.load(tmp_path / "st.npy", allow_pickle=False)
     assert saved.shape == (16, 8) and np.array_equal(saved, d1)
+
+
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-10), ("f32", 5e-5)])
+def test_cartesian_3d_small_vs_oracle(dtype, tol):
+    """The 3-D Cartesian stack (per-axis FFT lanes, strided and contiguous) at 32^3, 40 steps."""
+    kw = dict(n=32, steps=40, dt=0.01)
+    p, _ = W.cartesian_3d(so, dtype=dtype, **kw)
+    p.propagate()
+    po, _ = W.cartesian_3d(O, **kw)
+    po.propagate()
+    err = rel_l2(p.wave_function().array, po.wave_function().array)
+    assert err < tol, err
+    assert abs(p.wave_function().norm() - po.wave_function().norm()) < (1e-12 if dtype == "f64" else 2e-6)
+
+
+def test_cartesian_3d_mixed_radix_lanes():
+    """Lanes of 48 = 2^4 * 3 points along every axis (radix-3 passes), FP64 vs the oracle."""
+    kw = dict(n=48, steps=10, dt=0.01)
+    p, _ = W.cartesian_3d(so, dtype="f64", **kw)
+    p.propagate()
+    po, _ = W.cartesian_3d(O, **kw)
+    po.propagate()
+    assert rel_l2(p.wave_function().array, po.wave_function().array) < 1e-11
