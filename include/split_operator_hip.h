@@ -201,6 +201,23 @@ typedef struct {
 int so_propagation_time_passes(so_propagation *p, int64_t iters, int64_t cap, so_pass_timing *out, int64_t *n);
 int so_propagation_destroy(so_propagation *p);
 
+/* ---- batches (no crate counterpart: many Propagation instances stepped together) ----
+ * Every member walks its own stack exactly as so_propagation_step does; the k-th pass of all members runs as one
+ * kernel launch. Members share the batch's context and storage dtype and must not be reconfigured while in a batch. */
+typedef struct so_batch so_batch;
+int so_batch_create(so_context *ctx, so_batch **out);
+int so_batch_add(so_batch *b, so_propagation *p);      /* borrows p */
+int so_batch_remove(so_batch *b, so_propagation *p);
+int so_batch_size(so_batch *b, int64_t *n);
+int so_batch_step(so_batch *b, int64_t n);             /* steady steps replay a captured hipGraph */
+int so_batch_set_graphs(so_batch *b, int enable);
+/* launches every member's held-back work; reading one member alone would put it out of step with the others */
+int so_batch_flush(so_batch *b);
+int so_batch_stats(so_batch *b, int64_t *batched_launches, int64_t *graphs);
+int so_batch_destroy(so_batch *b);
+/* the last `count` LossSaver records (cumulative losses) of an op's loss checker, oldest first */
+int so_op_loss_tail(so_op *op, int64_t count, double *losses, int64_t *n);
+
 /* ---- helpers without a crate counterpart ---- */
 /* Gauss-Legendre nodes (ascending) and weights computed in FP64 on the GPU;
  * the crate's callers supply the polar grid of legendre_diagonalization.rs:13 */

@@ -347,6 +347,10 @@ class Op:
         lib().orc_op_loss_frames(self._h, _d(t), _d(l))
         return t[:n], l[:n]
 
+    def loss_tail(self, count):
+        """The last `count` cumulative losses the LossSaver recorded (oldest first)."""
+        return self.loss_frames()[1][-count:]
+
     def held_grid(self):
         return Grid(lib().orc_op_held_grid(self._h), owned=False)
 

@@ -239,6 +239,13 @@ class Op:
         call("so_op_loss_frames", self._h, dptr(t), dptr(l), n.value, C.byref(n))
         return t[: n.value], l[: n.value]
 
+    def loss_tail(self, count):
+        """The last `count` cumulative losses the LossSaver recorded (oldest first)."""
+        out = np.zeros(max(count, 1))
+        n = C.c_int64()
+        call("so_op_loss_tail", self._h, count, dptr(out), C.byref(n))
+        return out[: n.value]
+
     def held_grid(self):
         """The grid_transformation a transformation holds (swapped on transform)."""
         buf = C.create_string_buffer(256)
@@ -546,4 +553,46 @@ class Propagation:
     def __del__(self):
         if getattr(self, "_h", None) and _lib._lib is not None:
             _lib._lib.so_propagation_destroy(self._h)
+            self._h = None
+
+
+class Batch:
+    """Many propagations stepped together: the k-th pass of every member runs as one kernel launch."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        call("so_batch_create", context(), C.byref(h))
+        self._h = h
+        self._members = []
+
+    def add(self, prop):
+        prop._ready()
+        call("so_batch_add", self._h, prop._h)
+        self._members.append(prop)
+
+    def remove(self, prop):
+        call("so_batch_remove", self._h, prop._h)
+        self._members.remove(prop)
+
+    def __len__(self):
+        return len(self._members)
+
+    def steps(self, n):
+        call("so_batch_step", self._h, n)
+
+    def flush(self):
+        """Launch every member's held-back work (call before reading any member)."""
+        call("so_batch_flush", self._h)
+
+    def set_graphs(self, enable):
+        call("so_batch_set_graphs", self._h, 1 if enable else 0)
+
+    def stats(self):
+        a, g = C.c_int64(), C.c_int64()
+        call("so_batch_stats", self._h, C.byref(a), C.byref(g))
+        return {"batched_launches": a.value, "graphs": g.value}
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.so_batch_destroy(self._h)
             self._h = None

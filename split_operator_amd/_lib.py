@@ -93,6 +93,16 @@ SIGNATURES = {
     "so_propagation_destroy": [_vp],
     "so_propagation_time_passes": [_vp, C.c_int64, C.c_int64, _vp, _i64p],
     "so_gauss_legendre": [_vp, C.c_int64, _dp, _dp],
+    "so_batch_create": [_vp, _vpp],
+    "so_batch_add": [_vp, _vp],
+    "so_batch_remove": [_vp, _vp],
+    "so_batch_size": [_vp, _i64p],
+    "so_batch_step": [_vp, C.c_int64],
+    "so_batch_destroy": [_vp],
+    "so_batch_set_graphs": [_vp, C.c_int],
+    "so_batch_flush": [_vp],
+    "so_batch_stats": [_vp, _i64p, _i64p],
+    "so_op_loss_tail": [_vp, C.c_int64, _dp, _i64p],
 }
 
 

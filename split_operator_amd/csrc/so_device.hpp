@@ -146,6 +146,8 @@ extern "C++" {
 int sod_launch_pass(int dtype, const sod::PassDesc* d_desc, const sod::PassDesc& h_desc, hipStream_t s);
 int sod_launch_scalar(const sod::PassDesc* d_desc, hipStream_t s);
 int sod_pass_lds_bytes(int dtype, const sod::PassDesc& h);
+int sod_launch_batch(int dtype, int nx, int xf_type, const sod::PassDesc* const* d_descs, const int64_t* d_starts,
+                     int ninst, int64_t total_tiles, int ept, size_t lds, hipStream_t s);
 int sod_launch_state_density(int dtype, const void* psi, int ndim, const int64_t* shape, int axis,
                              const double* const* w, double* out, hipStream_t s);
 int sod_launch_density(int dtype, const void* psi, int64_t size, double* out, int64_t out_stride, hipStream_t s);

@@ -695,8 +695,8 @@ __device__ void sum_partials(const double* partials, int64_t nt, int tid, double
 
 // Workgroup 0 runs the scalar program the previous pass deferred (its norm
 // partials are visible here because a kernel boundary separates the passes).
-__device__ void run_inherited(const PassDesc* __restrict__ d, int tid) {
-  if (d->prev_nscalar == 0 || blockIdx.x != 0) return;
+__device__ void run_inherited(const PassDesc* __restrict__ d, int tid, int64_t tile) {
+  if (d->prev_nscalar == 0 || tile != 0) return;
   double acc[MAX_ACC];
   sum_partials(d->prev_partials, d->prev_ntiles, tid, acc, false);
   if (tid == 0) run_program(d->prev_sc, d->prev_nscalar, d->S, acc, d->nonfinite_slot);
@@ -705,10 +705,9 @@ __device__ void run_inherited(const PassDesc* __restrict__ d, int tid) {
 // Block reduction of the four accumulators into per-tile partials. FIN_DEFERRED
 // stops there (the next pass finishes the program); FIN_IMMEDIATE lets the last
 // workgroup to arrive (release/acquire on a counter) sum them and run the program.
-__device__ void finish_pass(const PassDesc* __restrict__ d, const Acc& in, int tid) {
+__device__ void finish_pass(const PassDesc* __restrict__ d, const Acc& in, int tid, int64_t tile, int64_t ntiles) {
   __shared__ double red[NTHREADS / 64][MAX_ACC];
   __shared__ int is_last;
-  const int64_t tile = blockIdx.x;
   if (!d->has_norm) {
     if (d->nscalar > 0 && tile == 0 && tid == 0) {
       double acc[MAX_ACC] = {0, 0, 0, 0};
@@ -736,7 +735,7 @@ __device__ void finish_pass(const PassDesc* __restrict__ d, const Acc& in, int t
   if (tid == 0) {
     __threadfence();
     const unsigned prev = atomicAdd(d->counter, 1u);
-    is_last = (prev == (unsigned)(gridDim.x - 1));
+    is_last = (prev == (unsigned)(ntiles - 1));
   }
   __syncthreads();
   if (!is_last) return;
@@ -754,10 +753,10 @@ __device__ void finish_pass(const PassDesc* __restrict__ d, const Acc& in, int t
 // Element-wise pass (no transform): a flat, coalesced sweep; each workgroup
 // owns NTHREADS * EPT consecutive elements.
 template <typename T, int EPT>
-__global__ __launch_bounds__(NTHREADS) void elem_pass_kernel(const PassDesc* __restrict__ d) {
+__device__ __forceinline__ void elem_pass_body(const PassDesc* __restrict__ d, int64_t tile, int64_t ntiles) {
   using T2 = typename CT<T>::T2;
   const int tid = threadIdx.x;
-  run_inherited(d, tid);
+  run_inherited(d, tid, tile);
   T2* psi = (T2*)d->psi;
   const int nd = d->ndim;
   int64_t stride[MAXD];
@@ -766,7 +765,7 @@ __global__ __launch_bounds__(NTHREADS) void elem_pass_kernel(const PassDesc* __r
     stride[a] = size;
     if (a < nd) size *= d->shape[a];
   }
-  const int64_t base = (int64_t)blockIdx.x * (NTHREADS * EPT);
+  const int64_t base = tile * (NTHREADS * EPT);
   T2 z[EPT];
 #pragma unroll
   for (int q = 0; q < EPT; ++q) {
@@ -829,7 +828,12 @@ __global__ __launch_bounds__(NTHREADS) void elem_pass_kernel(const PassDesc* __r
     const int64_t g = base + q * NTHREADS + tid;
     if (g < size) gst(&psi[g], z[q]);
   }
-  finish_pass(d, acc, tid);
+  finish_pass(d, acc, tid, tile, ntiles);
+}
+
+template <typename T, int EPT>
+__global__ __launch_bounds__(NTHREADS) void elem_pass_kernel(const PassDesc* __restrict__ d) {
+  elem_pass_body<T, EPT>(d, blockIdx.x, gridDim.x);
 }
 
 // LDS position of element k of a lane: padded in FFT passes, plain in matrix passes
@@ -869,13 +873,12 @@ __device__ __forceinline__ void map_global(int (&ql)[EPT], int (&qk)[EPT], int t
 // (psi, full-grid tables of later stages, 1-D tables and norm weights, GEMM B)
 // is issued at kernel start, so the memory latency is paid once.
 template <typename T, int KIND, int EPT>
-__global__ __launch_bounds__(NTHREADS) void xform_pass_kernel(const PassDesc* __restrict__ d) {
+__device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, int64_t tile, int64_t ntiles) {
   using T2 = typename CT<T>::T2;
   constexpr int PFN = EPT <= 8 ? MAX_PF : 0;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ LaneTab lt;
   const int tid = threadIdx.x;
-  const int64_t tile = blockIdx.x;
   const int tl = d->tl;
   const int n = (int)d->n;
   const int ld = d->ld;
@@ -946,7 +949,7 @@ __global__ __launch_bounds__(NTHREADS) void xform_pass_kernel(const PassDesc* __
           if (tid + q * NTHREADS < tot) pf[u][q] = gld(&tab[lt.base[ql[q]] + (int64_t)qk[q] * inner]);
       }
   }
-  run_inherited(d, tid);
+  run_inherited(d, tid, tile);
   Acc acc;
   stage_vec<T, EPT, PFN>(d->st[0], z, ql, qk, tid, tot, lt, d, acc, smem, pf);
 #pragma unroll
@@ -985,11 +988,47 @@ __global__ __launch_bounds__(NTHREADS) void xform_pass_kernel(const PassDesc* __
 #pragma unroll
   for (int q = 0; q < EPT; ++q)
     if (tid + q * NTHREADS < tot && lt.valid[ql[q]]) gst(&psi[lt.base[ql[q]] + (int64_t)qk[q] * inner], z[q]);
-  finish_pass(d, acc, tid);
+  finish_pass(d, acc, tid, tile, ntiles);
+}
+
+template <typename T, int KIND, int EPT>
+__global__ __launch_bounds__(NTHREADS) void xform_pass_kernel(const PassDesc* __restrict__ d) {
+  xform_pass_body<T, KIND, EPT>(d, blockIdx.x, gridDim.x);
+}
+
+// One launch for the same pass position of many independent wave functions
+// (a batch): workgroup b belongs to instance i with starts[i] <= b < starts[i+1].
+__device__ __forceinline__ int batch_instance(const int64_t* starts, int ninst) {
+  int lo = 0, hi = ninst;
+  const int64_t b = blockIdx.x;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (gld(&starts[mid]) <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <typename T, int KIND, int EPT>
+__global__ __launch_bounds__(NTHREADS) void xform_batch_kernel(const PassDesc* const* __restrict__ ds,
+                                                               const int64_t* __restrict__ starts, int ninst) {
+  const int i = batch_instance(starts, ninst);
+  const int64_t s0 = gld(&starts[i]);
+  xform_pass_body<T, KIND, EPT>(gld(&ds[i]), (int64_t)blockIdx.x - s0, gld(&starts[i + 1]) - s0);
+}
+
+template <typename T, int EPT>
+__global__ __launch_bounds__(NTHREADS) void elem_batch_kernel(const PassDesc* const* __restrict__ ds,
+                                                              const int64_t* __restrict__ starts, int ninst) {
+  const int i = batch_instance(starts, ninst);
+  const int64_t s0 = gld(&starts[i]);
+  elem_pass_body<T, EPT>(gld(&ds[i]), (int64_t)blockIdx.x - s0, gld(&starts[i + 1]) - s0);
 }
 
 __global__ __launch_bounds__(NTHREADS) void scalar_kernel(const PassDesc* __restrict__ d) {
-  run_inherited(d, threadIdx.x);
+  run_inherited(d, threadIdx.x, 0);
   __syncthreads();
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     double acc[MAX_ACC] = {0, 0, 0, 0};
@@ -1221,6 +1260,60 @@ int sod_launch_pass(int dtype, const PassDesc* d_desc, const PassDesc& h, hipStr
       else
         launch_xform<float, XF_FFT>(d_desc, h.ept, grid, lds, s);
     }
+  }
+  return (int)hipGetLastError();
+}
+
+template <typename T, int KIND>
+static void launch_xform_batch(const PassDesc* const* ds, const int64_t* starts, int ninst, int ept, dim3 grid,
+                               size_t lds, hipStream_t s) {
+  switch (ept) {
+    case 4: xform_batch_kernel<T, KIND, 4><<<grid, NTHREADS, lds, s>>>(ds, starts, ninst); break;
+    case 8: xform_batch_kernel<T, KIND, 8><<<grid, NTHREADS, lds, s>>>(ds, starts, ninst); break;
+    default: xform_batch_kernel<T, KIND, 16><<<grid, NTHREADS, lds, s>>>(ds, starts, ninst); break;
+  }
+}
+
+template <typename T, int KIND>
+static void set_lds_attr_batch() {
+  (void)hipFuncSetAttribute((const void*)xform_batch_kernel<T, KIND, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)SOD_MAX_DYN_LDS);
+  (void)hipFuncSetAttribute((const void*)xform_batch_kernel<T, KIND, 8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)SOD_MAX_DYN_LDS);
+  (void)hipFuncSetAttribute((const void*)xform_batch_kernel<T, KIND, 16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)SOD_MAX_DYN_LDS);
+}
+
+int sod_launch_batch(int dtype, int nx, int xf_type, const PassDesc* const* d_descs, const int64_t* d_starts,
+                     int ninst, int64_t total_tiles, int ept, size_t lds, hipStream_t s) {
+  const dim3 grid((unsigned)total_tiles);
+  if (nx == 0) {
+    if (dtype)
+      elem_batch_kernel<double, SOD_ELEM_EPT><<<grid, NTHREADS, 0, s>>>(d_descs, d_starts, ninst);
+    else
+      elem_batch_kernel<float, SOD_ELEM_EPT><<<grid, NTHREADS, 0, s>>>(d_descs, d_starts, ninst);
+    return (int)hipGetLastError();
+  }
+  static bool attr_set = false;
+  if (!attr_set) {
+    set_lds_attr_batch<float, XF_FFT>();
+    set_lds_attr_batch<float, XF_MAT>();
+    set_lds_attr_batch<double, XF_FFT>();
+    set_lds_attr_batch<double, XF_MAT>();
+    attr_set = true;
+  }
+  if (lds > SOD_MAX_DYN_LDS) return (int)hipErrorInvalidValue;
+  const bool mat = xf_type == XF_MAT;
+  if (dtype) {
+    if (mat)
+      launch_xform_batch<double, XF_MAT>(d_descs, d_starts, ninst, ept, grid, lds, s);
+    else
+      launch_xform_batch<double, XF_FFT>(d_descs, d_starts, ninst, ept, grid, lds, s);
+  } else {
+    if (mat)
+      launch_xform_batch<float, XF_MAT>(d_descs, d_starts, ninst, ept, grid, lds, s);
+    else
+      launch_xform_batch<float, XF_FFT>(d_descs, d_starts, ninst, ept, grid, lds, s);
   }
   return (int)hipGetLastError();
 }

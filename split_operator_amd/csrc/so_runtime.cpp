@@ -285,6 +285,7 @@ struct DevTable {
   }
 };
 
+struct Propagation;
 struct so_op {
   OpKind kind;
   so_context* ctx = nullptr;
@@ -314,6 +315,7 @@ struct so_op {
   int64_t per_frame = 0;
   std::vector<double> times;
   bool owned_by_stack = false;
+  Propagation* prop = nullptr;  // the propagation whose stack holds this op
 };
 
 // ============================================================== wave function
@@ -823,6 +825,10 @@ struct Propagation {
 
 static void flush_owner(so_wave_function* wf) {
   if (wf && wf->owner) wf->owner->flush();
+}
+// launches what a propagation still holds back, before one of its ops is read
+static void flush_op(so_op* op) {
+  if (op && op->prop) op->prop->flush();
 }
 
 // GEMM operand for a matrix transformation: B = M^T (real) or the real 2n x 2n
@@ -1722,6 +1728,7 @@ static int write_loss_file(so_op* op) {
 extern "C" int so_op_loss(so_op* op, double* out, int* present) {
   API_BEGIN
   NEED(op);
+  flush_op(op);
   NEED(out);
   if (present) *present = op->chk.present ? 1 : 0;
   if (!op->chk.present) {
@@ -1739,6 +1746,7 @@ extern "C" int so_op_loss(so_op* op, double* out, int* present) {
 extern "C" int so_op_loss_frames(so_op* op, double* times, double* losses, int64_t cap, int64_t* n) {
   API_BEGIN
   NEED(op);
+  flush_op(op);
   NEED(n);
   *n = 0;
   LossCheckerH& c = op->chk;
@@ -1983,6 +1991,7 @@ static std::vector<double> saver_host_frames(so_op* s) {
 extern "C" int so_saver_save(so_op* s) {
   API_BEGIN
   NEED(s);
+  flush_op(s);
   if (s->kind != OK_STATE_SAVER && s->kind != OK_WF_SAVER) fail(SO_ERR_INVALID, "not a saver");
   auto h = saver_host_frames(s);
   if (s->kind == OK_WF_SAVER) {
@@ -2013,6 +2022,7 @@ extern "C" int so_saver_reset(so_op* s) {
 extern "C" int so_saver_frames(so_op* s, double* data, double* times, int64_t* n_times) {
   API_BEGIN
   NEED(s);
+  flush_op(s);
   if (s->kind != OK_STATE_SAVER && s->kind != OK_WF_SAVER) fail(SO_ERR_INVALID, "not a saver");
   auto h = saver_host_frames(s);
   if (data) std::copy(h.begin(), h.end(), data);
@@ -2187,6 +2197,7 @@ extern "C" int so_propagation_set_operation_stack(so_propagation* pr, so_operati
   if (pr->stack) so_operation_stack_destroy(pr->stack);
   pr->stack = s;
   p.stack = s->items;
+  for (auto& it : p.stack) it.op->prop = &p;
   API_END
 }
 
@@ -2422,6 +2433,292 @@ extern "C" int so_propagation_destroy(so_propagation* pr) {
   }
   if (pr->stack) so_operation_stack_destroy(pr->stack);
   delete pr;
+  API_END
+}
+
+// ============================================================== C ABI: batches
+// A batch steps many independent propagations together: every member walks
+// its own stack, and the k-th pass of all members runs as one kernel launch
+// (workgroups find their member's descriptor from a prefix sum of tiles).
+struct BatchLaunch {
+  sod::PassDesc** d_descs = nullptr;
+  int64_t* d_starts = nullptr;
+  int64_t total = 0;
+  int ept = 4;
+  size_t lds = 0;
+  int nx = 0, xf_type = 0;
+};
+
+struct BatchItem {
+  BatchLaunch* bl = nullptr;  // one launch over every member
+  int member = -1;            // or one member's own launch
+  Launch L;
+};
+
+struct so_batch {
+  so_context* ctx = nullptr;
+  std::vector<so_propagation*> members;
+  std::map<std::string, BatchLaunch> cache;
+  struct GraphRec {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+  };
+  std::map<std::pair<std::string, int>, GraphRec> graphs;
+  bool use_graphs = true;
+  int64_t batched_launches = 0;
+  void drop_graphs() {
+    for (auto& kv : graphs) {
+      if (kv.second.ge) (void)hipGraphExecDestroy(kv.second.ge);
+      if (kv.second.g) (void)hipGraphDestroy(kv.second.g);
+    }
+    graphs.clear();
+  }
+  ~so_batch() {
+    drop_graphs();
+    for (auto& kv : cache) {
+      ctx->release(kv.second.d_descs);
+      ctx->release(kv.second.d_starts);
+    }
+  }
+};
+
+extern "C" int so_batch_create(so_context* ctx, so_batch** out) {
+  API_BEGIN
+  NEED(ctx);
+  NEED(out);
+  auto* b = new so_batch();
+  b->ctx = ctx;
+  *out = b;
+  API_END
+}
+
+extern "C" int so_batch_add(so_batch* b, so_propagation* pr) {
+  API_BEGIN
+  NEED(b);
+  NEED(pr);
+  if (pr->p.ctx != b->ctx) fail(SO_ERR_INVALID, "batch members must share the batch's context");
+  prop_ready(pr->p);
+  if (std::find(b->members.begin(), b->members.end(), pr) != b->members.end()) fail(SO_ERR_INVALID, "already a member");
+  if (!b->members.empty() && b->members[0]->p.dtype() != pr->p.dtype())
+    fail(SO_ERR_INVALID, "batch members must share the storage dtype");
+  b->ctx->sync();
+  b->drop_graphs();
+  b->members.push_back(pr);
+  API_END
+}
+
+extern "C" int so_batch_remove(so_batch* b, so_propagation* pr) {
+  API_BEGIN
+  NEED(b);
+  NEED(pr);
+  auto it = std::find(b->members.begin(), b->members.end(), pr);
+  if (it == b->members.end()) fail(SO_ERR_INVALID, "not a member");
+  b->ctx->sync();
+  b->drop_graphs();
+  b->members.erase(it);
+  API_END
+}
+
+extern "C" int so_batch_size(so_batch* b, int64_t* n) {
+  API_BEGIN
+  NEED(b);
+  NEED(n);
+  *n = (int64_t)b->members.size();
+  API_END
+}
+
+extern "C" int so_batch_set_graphs(so_batch* b, int enable) {
+  API_BEGIN
+  NEED(b);
+  b->use_graphs = enable != 0;
+  API_END
+}
+
+extern "C" int so_batch_stats(so_batch* b, int64_t* batched_launches, int64_t* graphs) {
+  API_BEGIN
+  NEED(b);
+  if (batched_launches) *batched_launches = b->batched_launches;
+  if (graphs) *graphs = (int64_t)b->graphs.size();
+  API_END
+}
+
+static BatchLaunch* batch_launch_for(so_batch* b, const std::vector<CompiledPass*>& passes) {
+  std::string key;
+  for (auto* cp : passes) key_add(key, &cp, sizeof(void*));
+  auto it = b->cache.find(key);
+  if (it != b->cache.end()) return &it->second;
+  BatchLaunch bl;
+  std::vector<sod::PassDesc*> ds;
+  std::vector<int64_t> starts(1, 0);
+  for (auto* cp : passes) {
+    ds.push_back(cp->d);
+    starts.push_back(starts.back() + cp->h.ntiles);
+    bl.ept = std::max(bl.ept, cp->h.ept);
+    bl.lds = std::max(bl.lds, (size_t)cp->h.lds_bytes);
+  }
+  bl.total = starts.back();
+  bl.nx = passes[0]->h.nx;
+  bl.xf_type = bl.nx ? passes[0]->h.xf[0].type : 0;
+  bl.d_descs = (sod::PassDesc**)b->ctx->alloc(sizeof(void*) * ds.size());
+  bl.d_starts = (int64_t*)b->ctx->alloc(sizeof(int64_t) * starts.size());
+  HIPCHK(hipMemcpy(bl.d_descs, ds.data(), sizeof(void*) * ds.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(bl.d_starts, starts.data(), sizeof(int64_t) * starts.size(), hipMemcpyHostToDevice));
+  return &b->cache.emplace(key, bl).first->second;
+}
+
+// Groups the k-th launch of every member into one launch when all members
+// emitted the same sequence of pass kinds this step; otherwise every member
+// launches its own passes.
+static std::vector<BatchItem> batch_plan(so_batch* b, std::vector<std::vector<Launch>>& ls) {
+  std::vector<BatchItem> plan;
+  const size_t M = ls.size();
+  const size_t K = ls[0].size();
+  bool same = M > 1;
+  for (auto& v : ls) same &= v.size() == K;
+  for (size_t k = 0; same && k < K; ++k) {
+    const Launch& L0 = ls[0][k];
+    for (auto& v : ls) {
+      const Launch& L = v[k];
+      if (L.is_save != L0.is_save)
+        same = false;
+      else if (!L.is_save && (L.pass->scalar_only != L0.pass->scalar_only || L.pass->h.nx != L0.pass->h.nx ||
+                              (L.pass->h.nx && L.pass->h.xf[0].type != L0.pass->h.xf[0].type)))
+        same = false;
+    }
+  }
+  if (!same) {
+    for (size_t i = 0; i < M; ++i)
+      for (auto& L : ls[i]) plan.push_back(BatchItem{nullptr, (int)i, L});
+    return plan;
+  }
+  for (size_t k = 0; k < K; ++k) {
+    if (ls[0][k].is_save || ls[0][k].pass->scalar_only) {
+      for (size_t i = 0; i < M; ++i) plan.push_back(BatchItem{nullptr, (int)i, ls[i][k]});
+      continue;
+    }
+    std::vector<CompiledPass*> ps;
+    for (auto& v : ls) ps.push_back(v[k].pass);
+    plan.push_back(BatchItem{batch_launch_for(b, ps), -1, Launch()});
+  }
+  return plan;
+}
+
+static void batch_run(so_batch* b, const std::vector<BatchItem>& plan) {
+  const int dt = b->members[0]->p.dtype();
+  for (const auto& it : plan) {
+    if (it.bl) {
+      const BatchLaunch& bl = *it.bl;
+      HIPCHK((hipError_t)sod_launch_batch(dt, bl.nx, bl.xf_type, bl.d_descs, bl.d_starts, (int)b->members.size(),
+                                          bl.total, bl.ept, bl.lds, b->ctx->stream));
+      b->batched_launches++;
+    } else {
+      b->members[it.member]->p.launch(it.L);
+    }
+  }
+}
+
+static std::string plan_key(const std::vector<BatchItem>& plan) {
+  std::string k;
+  for (const auto& it : plan) {
+    key_add(k, &it.bl, sizeof(void*));
+    key_add(k, &it.member, sizeof(int));
+    if (it.L.is_save) k.push_back('S');
+    key_add(k, &it.L.pass, sizeof(void*));
+  }
+  return k;
+}
+
+// Steps every member n times. Steps whose plan and end states repeat the
+// previous step's are replayed from a captured hipGraph of up to 64 steps
+// (not for members with savers on their stack, whose frames land on
+// particular steps).
+extern "C" int so_batch_step(so_batch* b, int64_t n) {
+  API_BEGIN
+  NEED(b);
+  if (b->members.empty() || n <= 0) return SO_OK;
+  bool graphs_ok = b->use_graphs;
+  for (auto* m : b->members)
+    for (auto& it : m->p.stack) graphs_ok &= it.type != 2;
+  const size_t M = b->members.size();
+  std::vector<std::string> prev_sig(M);
+  for (size_t i = 0; i < M; ++i) prev_sig[i] = b->members[i]->p.state_sig();
+  std::string prev_key;
+  std::vector<BatchItem> prev_plan;
+  bool steady = false;
+  std::vector<std::vector<Launch>> ls(M);
+  int64_t done = 0;
+  while (done < n) {
+    if (steady && graphs_ok) {
+      const int64_t K = std::min<int64_t>(64, n - done);
+      if (K >= 2) {
+        auto gkey = std::make_pair(prev_key, (int)K);
+        auto git = b->graphs.find(gkey);
+        if (git == b->graphs.end()) {
+          so_batch::GraphRec rec;
+          HIPCHK(hipStreamBeginCapture(b->ctx->stream, hipStreamCaptureModeThreadLocal));
+          for (int64_t s = 0; s < K; ++s) batch_run(b, prev_plan);
+          HIPCHK(hipStreamEndCapture(b->ctx->stream, &rec.g));
+          HIPCHK(hipGraphInstantiate(&rec.ge, rec.g, nullptr, nullptr, 0));
+          git = b->graphs.emplace(gkey, rec).first;
+        }
+        HIPCHK(hipGraphLaunch(git->second.ge, b->ctx->stream));
+        done += K;
+        continue;
+      }
+    }
+    bool sig_same = true;
+    bool has_save = false;
+    for (size_t i = 0; i < M; ++i) {
+      Propagation& p = b->members[i]->p;
+      p.sim_step();
+      ls[i].clear();
+      ls[i].swap(p.emitted);
+      for (auto& L : ls[i]) has_save |= L.is_save;
+      std::string sig = p.state_sig();
+      sig_same &= sig == prev_sig[i];
+      prev_sig[i].swap(sig);
+    }
+    std::vector<BatchItem> plan = batch_plan(b, ls);
+    batch_run(b, plan);
+    std::string key = plan_key(plan);
+    steady = !has_save && sig_same && key == prev_key && !plan.empty();
+    prev_key.swap(key);
+    prev_plan.swap(plan);
+    done++;
+  }
+  API_END
+}
+
+extern "C" int so_batch_flush(so_batch* b) {
+  API_BEGIN
+  NEED(b);
+  for (auto* m : b->members) m->p.flush();
+  API_END
+}
+
+extern "C" int so_batch_destroy(so_batch* b) {
+  API_BEGIN
+  if (!b) return SO_OK;
+  b->ctx->sync();
+  delete b;
+  API_END
+}
+
+// the last `count` LossSaver records of an op's checker (oldest first)
+extern "C" int so_op_loss_tail(so_op* op, int64_t count, double* losses, int64_t* n) {
+  API_BEGIN
+  NEED(op);
+  flush_op(op);
+  NEED(n);
+  *n = 0;
+  LossCheckerH& c = op->chk;
+  if (!c.present || !c.has_saver) return SO_OK;
+  const int64_t freq = 2 * c.total_frames / c.frames_no;
+  const int64_t calls = (int64_t)op->ctx->get_slot(c.s_cnt);
+  int64_t rec = std::min<int64_t>((calls + freq - 1) / freq, c.frames_no);
+  const int64_t k = std::min(count, rec);
+  if (k > 0 && losses) HIPCHK(hipMemcpy(losses, c.d_hist + (rec - k), sizeof(double) * k, hipMemcpyDeviceToHost));
+  *n = k;
   API_END
 }
 

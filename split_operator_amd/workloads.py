@@ -62,13 +62,15 @@ def radial_angular_params(seed):
 
 
 def radial_angular(B, n_r=2048, n_theta=128, r_min=0.5, r_max=50.0, dt=0.002, steps=2000, seed=1000,
-                   im_time=False, border=True, leak=False, dtype=None, gl=None, r0=8.0, sigma=1.0):
+                   im_time=False, border=True, leak=False, dtype=None, gl=None, r0=8.0, sigma=1.0,
+                   leak_frames=0, params=None):
     """2-D radial x angular propagation: r in [0.5, 50] (n_r points), theta on n_theta
     Gauss-Legendre nodes, V = C12/r^12 - C6/r^6 (1 + a P2(cos theta)), m = 1, Legendre
     transform to l = 0..n_theta-1 with the centrifugal term l(l+1)/(2 m r^2), radial FFT,
     border damping over the last 10% of r with its loss checked.
-    Stack [BorderDumping?, LeakControl?, V/2, Legendre, Rot/2, FFT_r, T]."""
-    c6, c12, a = radial_angular_params(seed)
+    Stack [BorderDumping?, LeakControl?, V/2, Legendre, Rot/2, FFT_r, T]. `leak_frames` > 0 gives the
+    LeakControl's loss checker a LossSaver of that many frames; `params` overrides the seeded (C6, C12, a)."""
+    c6, c12, a = radial_angular_params(seed) if params is None else params
     radial = B.Grid.new_linear_continuos("radial", r_min, r_max, n_r, 0)
     if gl is None:
         x, w = np.polynomial.legendre.leggauss(n_theta)
@@ -93,9 +95,13 @@ def radial_angular(B, n_r=2048, n_theta=128, r_min=0.5, r_max=50.0, dt=0.002, st
         bd = B.border_dumping(mask, radial)
         bd.add_loss_checker("border")
         p.add_control(bd, B.APPLY_FIRST | B.APPLY_SECOND)
+    lc = None
     if leak:
         lc = B.leak_control()
-        lc.add_loss_checker("leak")
+        if leak_frames > 0:
+            lc.add_loss_checker("leak", leak_frames, dt, steps)
+        else:
+            lc.add_loss_checker("leak")
         p.add_control(lc, B.APPLY_FIRST | B.APPLY_SECOND)
     p.add_propagator(B.n_dim_propagator(B.exp_propagator(V, B.select_step(False, dt, im_time))))
     t, ti = B.legendre_matrices(polar)
@@ -106,7 +112,101 @@ def radial_angular(B, n_r=2048, n_theta=128, r_min=0.5, r_max=50.0, dt=0.002, st
     p.add_transformation(B.fft_transformation(radial, "radial_momentum"))
     p.add_propagator(B.one_dim_propagator(B.exp_propagator(B.kinetic_hamiltonian(radial, 1.0),
                                                            B.select_step(True, dt, im_time)), 0))
+    p.leak = lc
     return p, (radial, polar), (c6, c12, a)
+
+
+RADIAL_SIZES = (512, 768, 1024, 1536, 2048)
+ANGULAR_SIZES = (32, 64, 128)
+
+
+def ground_state_params(seed, radial_sizes=RADIAL_SIZES, angular_sizes=ANGULAR_SIZES):
+    """Seeded instance of the batched ground-state workload: (N_r, N_theta, C6, C12, a), with
+    N_r uniform over `radial_sizes`, N_theta over `angular_sizes`, C6 in U[1, 3], C12 in U[0.5, 2],
+    a in U[0, 0.3]."""
+    rng = SplitMix64(seed)
+    n_r = radial_sizes[rng.next_u64() % len(radial_sizes)]
+    n_t = angular_sizes[rng.next_u64() % len(angular_sizes)]
+    c6 = rng.uniform(1.0, 3.0)
+    c12 = rng.uniform(0.5, 2.0)
+    a = rng.uniform(0.0, 0.3)
+    return n_r, n_t, (c6, c12, a)
+
+
+def decay_energy(l_prev, l_cur, dt):
+    """Energy from one imaginary-time step of a renormalised state: the LeakControl loss of the step
+    is 1 - N1/N0 with N1/N0 = exp(-2 E dt), so E = -ln(1 - loss) / (2 dt) (the crate's decay
+    estimator; `l_prev`, `l_cur` are consecutive cumulative losses)."""
+    return -math.log1p(-(l_cur - l_prev)) / (2.0 * dt)
+
+
+class GroundStateBatch:
+    """Imaginary-time relaxation of many radial x angular instances at once (uneven grids, each with
+    its own seeded potential). Every instance runs the stack [LeakControl (FP64 renormalisation each
+    step, loss saved every step), V/2, Legendre, Rot/2, FFT_r, T]; the per-step energy is the decay
+    estimator of consecutive LeakControl losses. An instance retires from the batch once its relative
+    energy change per step falls below `tol`.
+
+    `B` must provide Batch (the HIP backend); `batched=False` steps the same instances one by one."""
+
+    def __init__(self, B, count=64, seed0=4000, dt=0.05, max_steps=40000, tol=1e-10, dtype="f64",
+                 sizes=None, batched=True, r0=8.0, sigma=1.0):
+        self.B, self.dt, self.max_steps, self.tol = B, dt, max_steps, tol
+        self.inst = []
+        for i in range(count):
+            if sizes is None:
+                n_r, n_t, prm = ground_state_params(seed0 + i)
+            else:
+                n_r, n_t, prm = ground_state_params(seed0 + i, *sizes)
+            p, grids, prm = radial_angular(B, n_r=n_r, n_theta=n_t, dt=dt, steps=max_steps, im_time=True,
+                                           border=False, leak=True, dtype=dtype, leak_frames=2 * max_steps,
+                                           params=prm, r0=r0, sigma=sigma)
+            self.inst.append({"prop": p, "n_r": n_r, "n_theta": n_t, "params": prm, "steps": 0,
+                              "energy": float("nan"), "converged": False, "de": float("nan")})
+        self.batched = batched
+        self.batch = None
+        if batched:
+            self.batch = B.Batch()
+            for it in self.inst:
+                self.batch.add(it["prop"])
+        self.active = list(self.inst)
+        self.steps_done = 0
+
+    def points(self, active_only=True):
+        src = self.active if active_only else self.inst
+        return sum(it["n_r"] * it["n_theta"] for it in src)
+
+    def advance(self, k):
+        """k imaginary-time steps of every active instance."""
+        if self.batched:
+            self.batch.steps(k)
+            self.batch.flush()
+        else:
+            for it in self.active:
+                it["prop"].steps(k)
+        self.steps_done += k
+
+    def check(self):
+        """Reads each active instance's last energies; retires the converged ones."""
+        for it in list(self.active):
+            tail = it["prop"].leak.loss_tail(3)
+            it["steps"] = self.steps_done
+            if len(tail) < 3:
+                continue
+            e_prev = decay_energy(tail[0], tail[1], self.dt)
+            e = decay_energy(tail[1], tail[2], self.dt)
+            it["energy"], it["de"] = e, abs(e - e_prev) / abs(e)
+            if it["de"] < self.tol:
+                it["converged"] = True
+                self.active.remove(it)
+                if self.batched:
+                    self.batch.remove(it["prop"])
+
+    def run(self, check_every=256):
+        while self.active and self.steps_done < self.max_steps:
+            self.advance(min(check_every, self.max_steps - self.steps_done))
+            self.check()
+        return self.inst
 
 
 def cartesian_3d_params(seed, bumps=8, box=15.0):
