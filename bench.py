@@ -37,6 +37,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=200)
     ap.add_argument("--c3-steps", type=int, default=100, help="steps of the 256^3 Cartesian workload (0: skip)")
+    ap.add_argument("--c4-instances", type=int, default=64, help="batched ground-state instances (0: skip)")
+    ap.add_argument("--c4-max-steps", type=int, default=40000)
+    ap.add_argument("--c4-f32-steps", type=int, default=2000, help="steps of the FP32 batched throughput run")
+    ap.add_argument("--c4-solo", action="store_true", help="also relax the instances one by one")
     return ap.parse_args()
 
 
@@ -74,6 +78,67 @@ def cartesian_256(so, W, steps, kernel_iters):
     return {"value": pts * steps / wall, "unit": "grid-point-steps/s", "ms_per_step": wall / steps * 1e3,
             "steps": steps, "alg_bytes_per_step": step_bytes,
             "step_gbs": step_bytes / (wall / steps) / 1e9, "passes": passes}
+
+
+def ground_states(so, W, count, max_steps, f32_steps, solo):
+    """Batched imaginary-time ground states: `count` seeded radial x angular instances with uneven
+    grids (N_r in {512, 768, 1024, 1536, 2048}, N_theta in {32, 64, 128}), each with its own
+    (C6, C12, a), relaxed with FP64 storage and FP64 renormalisation every step until the relative
+    change of the decay-estimator energy per step is below 1e-10 (imaginary dt = 0.05)."""
+    dt, tol = 0.05, 1e-10
+
+    def relax(batched):
+        gs = W.GroundStateBatch(so, count=count, seed0=4000, dt=dt, max_steps=max_steps, tol=tol, dtype="f64",
+                                batched=batched)
+        so.synchronize()
+        t0 = time.perf_counter()
+        last = [0]
+
+        def progress():
+            if gs.steps_done - last[0] >= 4096:
+                last[0] = gs.steps_done
+                print(f"ground states: {gs.steps_done} steps, {len(gs.active)} active", file=sys.stderr, flush=True)
+
+        while gs.active and gs.steps_done < gs.max_steps:
+            gs.advance(min(256, gs.max_steps - gs.steps_done))
+            gs.check()
+            progress()
+        so.synchronize()
+        wall = time.perf_counter() - t0
+        pt_steps = sum(it["n_r"] * it["n_theta"] * it["steps"] for it in gs.inst)
+        out = {"time_to_ground_state_s": wall, "converged": sum(it["converged"] for it in gs.inst),
+               "steps_max": max(it["steps"] for it in gs.inst), "steps_min": min(it["steps"] for it in gs.inst),
+               "pt_steps_per_s": pt_steps / wall}
+        return out, gs
+
+    rec, gs = relax(True)
+    rec.update({"instances": count, "dt": dt, "tol": tol, "storage": "complex f64",
+                "energy_estimator": "decay: E = -ln(1 - LeakControl loss of the step) / (2 dt)",
+                "points": gs.points(active_only=False),
+                "energies": [it["energy"] for it in gs.inst],
+                "sizes": [[it["n_r"], it["n_theta"]] for it in gs.inst]})
+    energies64 = [it["energy"] for it in gs.inst]
+    del gs
+    if solo:
+        srec, sgs = relax(False)
+        rec["solo"] = srec
+        rec["solo_max_energy_diff"] = max(abs(a["energy"] - b) for a, b in zip(sgs.inst, energies64))
+        del sgs
+    # FP32 storage: batched throughput over a fixed number of steps, and how far its decay
+    # estimator lands from the converged FP64 energies after those steps
+    g32 = W.GroundStateBatch(so, count=count, seed0=4000, dt=dt, max_steps=f32_steps, tol=0.0, dtype="f32")
+    g32.advance(16)
+    so.synchronize()
+    t0 = time.perf_counter()
+    g32.advance(f32_steps - 16)
+    so.synchronize()
+    wall = time.perf_counter() - t0
+    g32.check()
+    errs = [abs(it["energy"] - e) / abs(e) for it, e in zip(g32.inst, energies64)]
+    rec["f32"] = {"steps": f32_steps - 16, "pt_steps_per_s": g32.points() * (f32_steps - 16) / wall,
+                  "ms_per_step": wall / (f32_steps - 16) * 1e3,
+                  "max_rel_energy_err_vs_f64_converged": max(errs)}
+    return rec
 
 
 def main():
@@ -145,6 +210,9 @@ def main():
     extra = {}
     if args.c3_steps > 0:
         extra["cartesian_3d_256"] = cartesian_256(so, W, args.c3_steps, args.kernel_iters)
+    if args.c4_instances > 0:
+        extra["ground_states_batched"] = ground_states(so, W, args.c4_instances, args.c4_max_steps,
+                                                       args.c4_f32_steps, args.c4_solo)
 
     line = {
         "metric": METRIC, "value": value, "unit": "grid-point-steps/s", "n_gpus": 1, "steps": args.steps,

@@ -203,7 +203,8 @@ int so_propagation_destroy(so_propagation *p);
 
 /* ---- batches (no crate counterpart: many Propagation instances stepped together) ----
  * Every member walks its own stack exactly as so_propagation_step does; the k-th pass of all members runs as one
- * kernel launch. Members share the batch's context and storage dtype and must not be reconfigured while in a batch. */
+ * kernel launch. Members share the batch's context and storage dtype and must not be reconfigured while in a batch;
+ * remove a member before destroying it. Joining a batch re-sizes a member's tiles for throughput. */
 typedef struct so_batch so_batch;
 int so_batch_create(so_context *ctx, so_batch **out);
 int so_batch_add(so_batch *b, so_propagation *p);      /* borrows p */

@@ -338,12 +338,35 @@ struct KChunk {
 };
 template <typename T>
 struct BFast {
-  static constexpr int KS = sizeof(T) == 4 ? 32 : 16;  // k-steps of the prefetched B fragment
-  static constexpr int TPW = 2;                         // 16x16 output tiles per wave
+  static constexpr int KS = 32;                        // k-steps of a B fragment held in registers (K <= 128)
+  static constexpr int TPW = sizeof(T) == 4 ? 2 : 1;   // column blocks a wave holds at once
 };
 
-// B fragments of one GEMM for the tiles a wave owns, loaded into registers
-// ahead of use (fast path: at most TPW tiles per wave and kpad <= 4 KS).
+// Work split of the register-B path: a wave owns column blocks col0,
+// col0 + cstep, ... (TPW at a time) and, for each, the row tiles row0,
+// row0 + rstep, ... When there are fewer column blocks than waves, the
+// waves sharing a column block split its row tiles.
+struct MatPlan {
+  int col0, cstep, row0, rstep;
+};
+__device__ __forceinline__ MatPlan mat_plan(int ctiles, int wave) {
+  constexpr int NW = NTHREADS / 64;
+  MatPlan p;
+  if (ctiles >= NW) {
+    p.col0 = wave;
+    p.cstep = NW;
+    p.row0 = 0;
+    p.rstep = 1;
+  } else {
+    const int g = NW / ctiles;
+    p.col0 = wave < ctiles * g ? wave % ctiles : ctiles;  // waves beyond ctiles * g stay idle
+    p.cstep = ctiles;
+    p.row0 = wave / ctiles;
+    p.rstep = g;
+  }
+  return p;
+}
+
 template <typename T>
 struct BRegs {
   T b[BFast<T>::TPW][BFast<T>::KS];
@@ -351,29 +374,35 @@ struct BRegs {
 
 template <typename T>
 __device__ __forceinline__ bool mat_fast(const XformDesc& x, int tl) {
-  const int rows = x.cplx ? tl : 2 * tl;
-  const int tiles = ((rows + 15) / 16) * (x.npad / 16);
-  return tiles <= BFast<T>::TPW * (NTHREADS / 64) && x.kpad <= 4 * BFast<T>::KS;
+  (void)tl;
+  return x.kpad <= 4 * BFast<T>::KS;
+}
+
+// B fragments of the column blocks round `v0` of this wave (v0 = 0 at
+// kernel start, so their latency overlaps the psi loads)
+template <typename T>
+__device__ __forceinline__ void mat_load_b(const XformDesc& x, int tid, int v0, BRegs<T>& br) {
+  const int ctiles = x.npad / 16;
+  const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const MatPlan pl = mat_plan(ctiles, wave);
+  const T* __restrict__ B = (const T*)x.B;
+#pragma unroll
+  for (int u = 0; u < BFast<T>::TPW; ++u) {
+    const int ct = pl.col0 + pl.cstep * (v0 + u);
+    const bool has = ct < ctiles;
+    const T* bcol = B + (has ? ct : 0) * 16 + lr;
+#pragma unroll
+    for (int c = 0; c < BFast<T>::KS; ++c) {
+      const int k = 4 * c + lk;
+      br.b[u][c] = (has && k < x.kpad) ? gld(&bcol[(int64_t)k * x.npad]) : (T)0;
+    }
+  }
 }
 
 template <typename T>
 __device__ __forceinline__ void mat_prefetch(const XformDesc& x, int tl, int tid, BRegs<T>& br) {
-  const int rows = x.cplx ? tl : 2 * tl;
-  const int rtiles = (rows + 15) / 16;
-  const int ctiles = x.npad / 16;
-  const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
-  const T* __restrict__ B = (const T*)x.B;
-#pragma unroll
-  for (int u = 0; u < BFast<T>::TPW; ++u) {
-    const int t = wave + u * (NTHREADS / 64);
-    const int ct = t < rtiles * ctiles ? t % ctiles : 0;
-    const T* bcol = B + ct * 16 + lr;
-#pragma unroll
-    for (int c = 0; c < BFast<T>::KS; ++c) {
-      const int k = 4 * c + lk;
-      br.b[u][c] = (t < rtiles * ctiles && k < x.kpad) ? gld(&bcol[(int64_t)k * x.npad]) : (T)0;
-    }
-  }
+  (void)tl;
+  mat_load_b<T>(x, tid, 0, br);
 }
 
 template <typename T, typename T2, typename ACC>
@@ -396,11 +425,12 @@ __device__ __forceinline__ void mat_store_tile(T* Cm, const ACC& acc, int rt, in
 // out-lanes = M . in-lanes for `tl` lanes in LDS. Real matrix: GEMM rows are
 // (lane, re|im) pairs, K = n, columns = n. Complex matrix: rows are lanes seen
 // as 2n reals, K = 2n, columns = 2n, B holds the real 2n x 2n form of M^T.
-// Each wave owns 16 x 16 output tiles. Fast path: B was prefetched into
-// registers; otherwise B and A are loaded per chunk of KC k-steps.
+// Fast path (K <= 128): a wave holds the B fragments of TPW column blocks in
+// registers and sweeps every row tile it owns with them, so B is read once
+// per workgroup. Otherwise B and A are loaded per chunk of KC k-steps.
 template <typename T, typename T2>
 __device__ __forceinline__ void mat_lds(const T2* in, T2* out, int tl, int ld, const XformDesc& x, int tid,
-                                        const BRegs<T>& br, bool fast) {
+                                        BRegs<T>& br, bool fast) {
   using M = Mfma<T>;
   const T* A = (const T*)in;
   T* Cm = (T*)out;
@@ -413,44 +443,48 @@ __device__ __forceinline__ void mat_lds(const T2* in, T2* out, int tl, int ld, c
   const int lk = lane >> 4;
   const int kdim = x.kdim, kpad = x.kpad, npad = x.npad, ncols = x.ncols;
   if (fast) {
-    // the (up to) two tiles of this wave run interleaved MFMA chains
-    const int t0 = wave, t1 = wave + NTHREADS / 64;
-    const int ntile = rtiles * ctiles;
-    const bool has0 = t0 < ntile, has1 = t1 < ntile;
-    const int rt0 = has0 ? t0 / ctiles : 0, ct0 = has0 ? t0 - rt0 * ctiles : 0;
-    const int rt1 = has1 ? t1 / ctiles : 0, ct1 = has1 ? t1 - rt1 * ctiles : 0;
+    constexpr int TPW = BFast<T>::TPW, KS = BFast<T>::KS;
+    const MatPlan pl = mat_plan(ctiles, wave);
     const int astride = x.cplx ? 1 : 2;
-    T av0[BFast<T>::KS], av1[BFast<T>::KS];
-    {
-      const int row = rt0 * 16 + lr;
-      const bool rv = has0 && row < rows;
-      const int abase = x.cplx ? row * ld * 2 : (row >> 1) * ld * 2 + (row & 1);
+    for (int v0 = 0; pl.col0 + pl.cstep * v0 < ctiles; v0 += TPW) {
+      if (v0 > 0) mat_load_b<T>(x, tid, v0, br);
+      int ct[TPW];
+      bool has[TPW];
 #pragma unroll
-      for (int c = 0; c < BFast<T>::KS; ++c) {
-        const int k = 4 * c + lk;
-        av0[c] = (rv && k < kdim) ? lld(&A[abase + k * astride]) : (T)0;
+      for (int u = 0; u < TPW; ++u) {
+        ct[u] = pl.col0 + pl.cstep * (v0 + u);
+        has[u] = ct[u] < ctiles;
+      }
+      for (int rt = pl.row0; rt < rtiles; rt += pl.rstep) {
+        const int row = rt * 16 + lr;
+        const bool rv = row < rows;
+        const int abase = x.cplx ? row * ld * 2 : (row >> 1) * ld * 2 + (row & 1);
+        typename M::acc_t acc[TPW];
+#pragma unroll
+        for (int u = 0; u < TPW; ++u) acc[u] = typename M::acc_t{0, 0, 0, 0};
+        // A fragments in chunks of AC k-steps keep few registers live
+        constexpr int AC = 8;
+#pragma unroll
+        for (int c0 = 0; c0 < KS; c0 += AC)
+          if (4 * c0 < kpad) {
+            T av[AC];
+#pragma unroll
+            for (int c = 0; c < AC; ++c) {
+              const int k = 4 * (c0 + c) + lk;
+              av[c] = (rv && k < kdim) ? lld(&A[abase + k * astride]) : (T)0;
+            }
+#pragma unroll
+            for (int c = 0; c < AC; ++c)
+              if (4 * (c0 + c) < kpad) {
+#pragma unroll
+                for (int u = 0; u < TPW; ++u) acc[u] = M::run(av[c], br.b[u][c0 + c], acc[u]);
+              }
+          }
+#pragma unroll
+        for (int u = 0; u < TPW; ++u)
+          if (has[u]) mat_store_tile<T, T2>(Cm, acc[u], rt, ct[u], rows, ncols, ld, x.cplx, lr, lk);
       }
     }
-    const bool same = rt1 == rt0;
-    if (!same) {
-      const int row = rt1 * 16 + lr;
-      const bool rv = has1 && row < rows;
-      const int abase = x.cplx ? row * ld * 2 : (row >> 1) * ld * 2 + (row & 1);
-#pragma unroll
-      for (int c = 0; c < BFast<T>::KS; ++c) {
-        const int k = 4 * c + lk;
-        av1[c] = (rv && k < kdim) ? lld(&A[abase + k * astride]) : (T)0;
-      }
-    }
-    typename M::acc_t acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
-#pragma unroll
-    for (int c = 0; c < BFast<T>::KS; ++c)
-      if (4 * c < kpad) {
-        acc0 = M::run(av0[c], br.b[0][c], acc0);
-        acc1 = M::run(same ? av0[c] : av1[c], br.b[1][c], acc1);
-      }
-    if (has0) mat_store_tile<T, T2>(Cm, acc0, rt0, ct0, rows, ncols, ld, x.cplx, lr, lk);
-    if (has1) mat_store_tile<T, T2>(Cm, acc1, rt1, ct1, rows, ncols, ld, x.cplx, lr, lk);
     return;
   }
   constexpr int KC = KChunk<T>::value;
@@ -657,17 +691,18 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Sums per-tile partials in a fixed order (strided per thread, then wave and
 // block trees); every thread of the block gets the four totals.
-__device__ void sum_partials(const double* partials, int64_t nt, int tid, double* out, bool nontemporal) {
+__device__ void sum_partials(const double* partials, int64_t nt, int tid, double* out, bool coherent) {
   __shared__ double red2[NTHREADS / 64][MAX_ACC];
   const int wave = tid >> 6, lane = tid & 63;
   double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
   for (int64_t t = tid; t < nt; t += NTHREADS) {
     const double* p = partials + t * MAX_ACC;
-    if (nontemporal) {
-      t0 += __builtin_nontemporal_load(p + 0);
-      t1 += __builtin_nontemporal_load(p + 1);
-      t2 += __builtin_nontemporal_load(p + 2);
-      t3 += __builtin_nontemporal_load(p + 3);
+    if (coherent) {
+      // written by other workgroups of this launch with device-coherent stores
+      t0 += __hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      t1 += __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      t2 += __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      t3 += __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       t0 += gld(p + 0);
       t1 += gld(p + 1);
@@ -703,8 +738,14 @@ __device__ void run_inherited(const PassDesc* __restrict__ d, int tid, int64_t t
 }
 
 // Block reduction of the four accumulators into per-tile partials. FIN_DEFERRED
-// stops there (the next pass finishes the program); FIN_IMMEDIATE lets the last
-// workgroup to arrive (release/acquire on a counter) sum them and run the program.
+// stops there (the next pass finishes the program; the kernel boundary makes
+// the partials visible). FIN_IMMEDIATE lets the last workgroup to arrive sum
+// them and run the program. No memory fence is used for that: a device-scope
+// fence writes back every dirty L2 line of the XCD (all of psi this pass
+// wrote) and would serialise the workgroups. Instead the partials are written
+// and read with device-coherent (agent-scope) atomic accesses, and each
+// workgroup waits for its partial stores to complete before it bumps the
+// arrival counter, so the last arrival reads complete partials.
 __device__ void finish_pass(const PassDesc* __restrict__ d, const Acc& in, int tid, int64_t tile, int64_t ntiles) {
   __shared__ double red[NTHREADS / 64][MAX_ACC];
   __shared__ int is_last;
@@ -715,6 +756,7 @@ __device__ void finish_pass(const PassDesc* __restrict__ d, const Acc& in, int t
     }
     return;
   }
+  const bool immediate = d->fin_mode == FIN_IMMEDIATE;
   const int wave = tid >> 6, lane = tid & 63;
   double a0 = wave_sum(in.a0), a1 = wave_sum(in.a1), a2 = wave_sum(in.a2), a3 = wave_sum(in.a3);
   if (lane == 0) {
@@ -728,25 +770,38 @@ __device__ void finish_pass(const PassDesc* __restrict__ d, const Acc& in, int t
     for (int j = 0; j < MAX_ACC; ++j) {
       double s = 0.0;
       for (int w = 0; w < NTHREADS / 64; ++w) s += red[w][j];
-      gst(&d->partials[tile * MAX_ACC + j], s);
+      if (immediate)
+        __hip_atomic_store(&d->partials[tile * MAX_ACC + j], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      else
+        gst(&d->partials[tile * MAX_ACC + j], s);
     }
   }
-  if (d->fin_mode != FIN_IMMEDIATE) return;
+  if (!immediate) return;
   if (tid == 0) {
-    __threadfence();
-    const unsigned prev = atomicAdd(d->counter, 1u);
+    // the partial stores above are complete (acknowledged at the device
+    // coherence point) before the counter moves
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    const unsigned prev = __hip_atomic_fetch_add(d->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     is_last = (prev == (unsigned)(ntiles - 1));
   }
   __syncthreads();
   if (!is_last) return;
-  __threadfence();
   double acc[MAX_ACC];
   sum_partials(d->partials, d->ntiles, tid, acc, true);
   if (tid == 0) {
     run_scalar_program(d, acc);
-    *d->counter = 0u;
-    __threadfence();
+    __hip_atomic_store(d->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin by index; this mapping gives
+// each XCD a contiguous range of tiles, so tiles that share cache lines
+// (neighbouring strided lanes) are served by one L2.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t total) {
+  const int64_t per = total / NXCD, rem = total % NXCD;
+  const int64_t x = b % NXCD, j = b / NXCD;
+  return x * per + (x < rem ? x : rem) + j;
 }
 
 // ---------------------------------------------------------------- the pass kernels
@@ -833,7 +888,7 @@ __device__ __forceinline__ void elem_pass_body(const PassDesc* __restrict__ d, i
 
 template <typename T, int EPT>
 __global__ __launch_bounds__(NTHREADS) void elem_pass_kernel(const PassDesc* __restrict__ d) {
-  elem_pass_body<T, EPT>(d, blockIdx.x, gridDim.x);
+  elem_pass_body<T, EPT>(d, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
 }
 
 // LDS position of element k of a lane: padded in FFT passes, plain in matrix passes
@@ -993,14 +1048,14 @@ __device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, 
 
 template <typename T, int KIND, int EPT>
 __global__ __launch_bounds__(NTHREADS) void xform_pass_kernel(const PassDesc* __restrict__ d) {
-  xform_pass_body<T, KIND, EPT>(d, blockIdx.x, gridDim.x);
+  xform_pass_body<T, KIND, EPT>(d, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
 }
 
 // One launch for the same pass position of many independent wave functions
-// (a batch): workgroup b belongs to instance i with starts[i] <= b < starts[i+1].
-__device__ __forceinline__ int batch_instance(const int64_t* starts, int ninst) {
+// (a batch): tile b (after the XCD mapping) belongs to instance i with
+// starts[i] <= b < starts[i+1].
+__device__ __forceinline__ int batch_instance(const int64_t* starts, int ninst, int64_t b) {
   int lo = 0, hi = ninst;
-  const int64_t b = blockIdx.x;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
     if (gld(&starts[mid]) <= b)
@@ -1014,17 +1069,19 @@ __device__ __forceinline__ int batch_instance(const int64_t* starts, int ninst) 
 template <typename T, int KIND, int EPT>
 __global__ __launch_bounds__(NTHREADS) void xform_batch_kernel(const PassDesc* const* __restrict__ ds,
                                                                const int64_t* __restrict__ starts, int ninst) {
-  const int i = batch_instance(starts, ninst);
+  const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
+  const int i = batch_instance(starts, ninst, b);
   const int64_t s0 = gld(&starts[i]);
-  xform_pass_body<T, KIND, EPT>(gld(&ds[i]), (int64_t)blockIdx.x - s0, gld(&starts[i + 1]) - s0);
+  xform_pass_body<T, KIND, EPT>(gld(&ds[i]), b - s0, gld(&starts[i + 1]) - s0);
 }
 
 template <typename T, int EPT>
 __global__ __launch_bounds__(NTHREADS) void elem_batch_kernel(const PassDesc* const* __restrict__ ds,
                                                               const int64_t* __restrict__ starts, int ninst) {
-  const int i = batch_instance(starts, ninst);
+  const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
+  const int i = batch_instance(starts, ninst, b);
   const int64_t s0 = gld(&starts[i]);
-  elem_pass_body<T, EPT>(gld(&ds[i]), (int64_t)blockIdx.x - s0, gld(&starts[i + 1]) - s0);
+  elem_pass_body<T, EPT>(gld(&ds[i]), b - s0, gld(&starts[i + 1]) - s0);
 }
 
 __global__ __launch_bounds__(NTHREADS) void scalar_kernel(const PassDesc* __restrict__ d) {

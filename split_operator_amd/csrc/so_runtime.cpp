@@ -532,6 +532,9 @@ struct Propagation {
   std::unordered_map<std::string, std::unique_ptr<CompiledPass>> cache;
   // graphs
   bool use_graphs = true;
+  // member of a batch: tiles are sized for throughput (the batch as a whole
+  // supplies the workgroups) rather than for filling the GPU on their own
+  bool batch_mode = false;
   struct GraphRec {
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
@@ -1003,7 +1006,7 @@ CompiledPass* Propagation::compile(const PassB& b, bool immediate) {
       // tiles of at most 2048 elements (8 per thread) keep two or more
       // workgroups resident per CU, so one tile's memory phase overlaps
       // another's transform; halve further while fewer than 512 tiles
-      while (tl > min_tl && (tiles_for(tl) < 512 || (int64_t)tl * n > 2048)) tl /= 2;
+      while (tl > min_tl && ((!batch_mode && tiles_for(tl) < 512) || (int64_t)tl * n > 2048)) tl /= 2;
       if ((int64_t)tl * n > (int64_t)sod::FFT_EPT * sod::NTHREADS || lds_for(tl) > SOD_MAX_DYN_LDS)
         fail(SO_ERR_UNSUPPORTED, "lane length %lld exceeds the LDS tile limit for this dtype", (long long)n);
       h.ld = (int)lds_pitch(mat, n, tl);
@@ -2473,12 +2476,18 @@ struct so_batch {
     }
     graphs.clear();
   }
-  ~so_batch() {
-    drop_graphs();
+  // batched launches point at the members' compiled passes: dropped whenever
+  // a member joins or leaves (its passes are recompiled then)
+  void drop_launches() {
     for (auto& kv : cache) {
       ctx->release(kv.second.d_descs);
       ctx->release(kv.second.d_starts);
     }
+    cache.clear();
+  }
+  ~so_batch() {
+    drop_graphs();
+    drop_launches();
   }
 };
 
@@ -2503,6 +2512,9 @@ extern "C" int so_batch_add(so_batch* b, so_propagation* pr) {
     fail(SO_ERR_INVALID, "batch members must share the storage dtype");
   b->ctx->sync();
   b->drop_graphs();
+  b->drop_launches();
+  prop_reset_compiled(pr->p);
+  pr->p.batch_mode = true;
   b->members.push_back(pr);
   API_END
 }
@@ -2515,7 +2527,10 @@ extern "C" int so_batch_remove(so_batch* b, so_propagation* pr) {
   if (it == b->members.end()) fail(SO_ERR_INVALID, "not a member");
   b->ctx->sync();
   b->drop_graphs();
+  b->drop_launches();
   b->members.erase(it);
+  prop_reset_compiled(pr->p);
+  pr->p.batch_mode = false;
   API_END
 }
 
