@@ -219,6 +219,26 @@ int so_batch_destroy(so_batch *b);
 /* the last `count` LossSaver records (cumulative losses) of an op's loss checker, oldest first */
 int so_op_loss_tail(so_op *op, int64_t count, double *losses, int64_t *n);
 
+/* ---- warm-start network (no crate counterpart) ----
+ * A fully connected tanh MLP (widths[0] inputs, widths[nlayers] linear outputs) predicting base + net(x). Weights are
+ * FP32 masters with BF16 copies; every GEMM multiplies BF16 operands on MFMA with FP32 sums. x, target and y are device
+ * pointers to row-major FP32 [batch][width] arrays; parameters and gradients cross as host FP32 arrays (W is
+ * [widths[l+1]][widths[l]]). Calls run on the context's stream. */
+typedef struct so_mlp so_mlp;
+int so_mlp_create(so_context *ctx, int nlayers, const int64_t *widths, uint64_t seed, so_mlp **out);
+int so_mlp_destroy(so_mlp *m);
+int so_mlp_set_base(so_mlp *m, const float *base);                              /* host, widths[nlayers] values */
+int so_mlp_params(so_mlp *m, int layer, float *w, float *b);                    /* read FP32 masters (host) */
+int so_mlp_set_params(so_mlp *m, int layer, const float *w, const float *b);
+int so_mlp_forward(so_mlp *m, const float *x, int64_t batch, float *y);         /* y = base + net(x) */
+/* gradients of mean((base + net(x) - target)^2) into the network's gradient buffers; *loss = that mean */
+int so_mlp_backward(so_mlp *m, const float *x, const float *target, int64_t batch, double *loss);
+int so_mlp_gradient(so_mlp *m, int layer, float *gw, float *gb);                /* read gradients (host) */
+int so_mlp_adam(so_mlp *m, double lr, double beta1, double beta2, double eps);  /* one Adam step on the gradients */
+int so_mlp_train_step(so_mlp *m, const float *x, const float *target, int64_t batch, double lr, double *loss);
+/* C = op(A) op(B), FP32 device operands multiplied as BF16 (A is [M][K], or [K][M] when tra; B is [K][N], or [N][K]) */
+int so_mlp_gemm(so_mlp *m, const float *A, const float *B, float *C, int64_t M, int64_t N, int64_t K, int tra, int trb);
+
 /* ---- helpers without a crate counterpart ---- */
 /* Gauss-Legendre nodes (ascending) and weights computed in FP64 on the GPU;
  * the crate's callers supply the polar grid of legendre_diagonalization.rs:13 */

@@ -10,6 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_build", "libsplit_operator_hip.so")
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _i64p = C.POINTER(C.c_int64)
 _vp = C.c_void_p
 _vpp = C.POINTER(C.c_void_p)
@@ -103,6 +104,17 @@ SIGNATURES = {
     "so_batch_flush": [_vp],
     "so_batch_stats": [_vp, _i64p, _i64p],
     "so_op_loss_tail": [_vp, C.c_int64, _dp, _i64p],
+    "so_mlp_create": [_vp, C.c_int, _i64p, C.c_uint64, _vpp],
+    "so_mlp_destroy": [_vp],
+    "so_mlp_set_base": [_vp, _fp],
+    "so_mlp_params": [_vp, C.c_int, _fp, _fp],
+    "so_mlp_set_params": [_vp, C.c_int, _fp, _fp],
+    "so_mlp_forward": [_vp, _vp, C.c_int64, _vp],
+    "so_mlp_backward": [_vp, _vp, _vp, C.c_int64, _dp],
+    "so_mlp_gradient": [_vp, C.c_int, _fp, _fp],
+    "so_mlp_adam": [_vp, C.c_double, C.c_double, C.c_double, C.c_double],
+    "so_mlp_train_step": [_vp, _vp, _vp, C.c_int64, C.c_double, _dp],
+    "so_mlp_gemm": [_vp, _vp, _vp, _vp, C.c_int64, C.c_int64, C.c_int64, C.c_int, C.c_int],
 }
 
 

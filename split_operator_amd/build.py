@@ -1,7 +1,8 @@
 """Builds libsplit_operator_hip.so for gfx950 with hipcc (in-tree, no JIT cache).
 
-The library holds the HIP kernels (csrc/so_kernels.hip) and the host runtime
-with the C ABI of include/split_operator_hip.h (csrc/so_runtime.cpp).
+The library holds the propagation kernels (csrc/so_kernels.hip), the host
+runtime with the C ABI of include/split_operator_hip.h (csrc/so_runtime.cpp)
+and the warm-start network's kernels and ABI (csrc/so_mlp.hip).
 """
 import os
 import subprocess
@@ -12,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libsplit_operator_hip.so")
 ARCH = os.environ.get("SO_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["so_kernels.hip", "so_runtime.cpp"]
+SOURCES = ["so_kernels.hip", "so_runtime.cpp", "so_mlp.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

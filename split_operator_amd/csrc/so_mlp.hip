@@ -1,0 +1,543 @@
+// so_mlp.hip — the learned warm start's network on gfx950: a fully connected
+// tanh MLP whose GEMMs run on BF16 MFMA with FP32 accumulation, trained with
+// Adam entirely by the kernels below (forward, backward, loss, optimiser).
+//
+// Storage: FP32 master weights W[l] ([out][in], row-major) and biases, a BF16
+// copy of each W[l] refreshed by the optimiser step, FP32 activations. Every
+// GEMM operand is rounded to BF16 (round to nearest even) when it is staged
+// into LDS, so the products are BF16 x BF16 with FP32 sums.
+//
+// The network has no counterpart in the split_operator crate: it predicts a
+// ground-state guess psi(r, l) from potential parameters, and imaginary-time
+// propagation is started from that guess instead of a Gaussian.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "so_host.hpp"
+
+namespace {
+
+constexpr int GT = 64;        // output tile edge of a workgroup
+constexpr int GK = 32;        // k-depth of one LDS stage
+constexpr int GTHREADS = 256;  // 4 waves, each a 32 x 32 quarter of the tile
+constexpr int EW = 256;        // threads of the element-wise kernels
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16, round to nearest even (NaN stays NaN)
+__host__ __device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+template <typename T>
+__device__ __forceinline__ uint16_t as_bf(T v);
+template <>
+__device__ __forceinline__ uint16_t as_bf<float>(float v) {
+  return f2bf(v);
+}
+template <>
+__device__ __forceinline__ uint16_t as_bf<uint16_t>(uint16_t v) {
+  return v;
+}
+
+// C[split][m][n] = sum over k in split's chunk of A(m, k) B(k, n).
+// A(m, k) = TRA ? A[k lda + m] : A[m lda + k];  B(k, n) = TRB ? B[n ldb + k] : B[k ldb + n].
+// 16x16x16 BF16 MFMA: lane L supplies A[L%16][4(L/16)+i] and B[4(L/16)+i][L%16]
+// and holds D[4(L/16)+j][L%16].
+template <typename TA, typename TB, bool TRA, bool TRB>
+__global__ __launch_bounds__(GTHREADS) void gemm_bf16_kernel(const TA* __restrict__ A, int64_t lda,
+                                                             const TB* __restrict__ B, int64_t ldb, float* __restrict__ C,
+                                                             int64_t ldc, int64_t M, int64_t N, int64_t K,
+                                                             int64_t kchunk, int64_t split_stride) {
+  __shared__ __align__(16) uint16_t As[GT][GK + 4];  // [m][k]
+  __shared__ __align__(16) uint16_t Bs[GT][GK + 4];  // [n][k]
+  const int tid = threadIdx.x;
+  const int64_t m0 = (int64_t)blockIdx.y * GT, n0 = (int64_t)blockIdx.x * GT;
+  const int64_t kb = (int64_t)blockIdx.z * kchunk;
+  const int64_t ke = K < kb + kchunk ? K : kb + kchunk;
+  const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int64_t k0 = kb; k0 < ke; k0 += GK) {
+#pragma unroll
+    for (int i = 0; i < GT * GK / GTHREADS; ++i) {
+      const int e = tid + i * GTHREADS;
+      // consecutive threads walk the contiguous index of the source
+      const int mm = TRA ? e % GT : e / GK;
+      const int kk = TRA ? e / GT : e % GK;
+      const int64_t gm = m0 + mm, gk = k0 + kk;
+      uint16_t v = 0;
+      if (gm < M && gk < ke) v = as_bf<TA>(TRA ? A[gk * lda + gm] : A[gm * lda + gk]);
+      As[mm][kk] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < GT * GK / GTHREADS; ++i) {
+      const int e = tid + i * GTHREADS;
+      const int nn = TRB ? e / GK : e % GT;
+      const int kk = TRB ? e % GK : e / GT;
+      const int64_t gn = n0 + nn, gk = k0 + kk;
+      uint16_t v = 0;
+      if (gn < N && gk < ke) v = as_bf<TB>(TRB ? B[gn * ldb + gk] : B[gk * ldb + gn]);
+      Bs[nn][kk] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < GK; ks += 16) {
+      bf16x4 a[2], b[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        a[t] = *(const bf16x4*)&As[wm + t * 16 + lr][ks + 4 * lk];
+        b[t] = *(const bf16x4*)&Bs[wn + t * 16 + lr][ks + 4 * lk];
+      }
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[x], b[y], acc[x][y], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  float* Cs = C + (int64_t)blockIdx.z * split_stride;
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t gm = m0 + wm + x * 16 + 4 * lk + j, gn = n0 + wn + y * 16 + lr;
+        if (gm < M && gn < N) Cs[gm * ldc + gn] = acc[x][y][j];
+      }
+}
+
+// out[i] = sum_s part[s][i], in split order
+__global__ void splitk_reduce_kernel(const float* __restrict__ part, int64_t n, int splits, float* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int q = 0; q < splits; ++q) s += part[q * n + i];
+    out[i] = s;
+  }
+}
+
+// Z = act(Y + b) over [rows][cols]; act = tanh (hidden) or identity (output)
+// (in place: Y and Z may be the same buffer)
+__global__ void bias_act_kernel(const float* Y, const float* __restrict__ b, int64_t rows, int64_t cols, int use_tanh,
+                                float* Z) {
+  const int64_t n = rows * cols;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = Y[i] + b[i % cols];
+    Z[i] = use_tanh ? tanhf(v) : v;
+  }
+}
+
+// out = base + Z (the prediction)
+__global__ void add_base_kernel(const float* __restrict__ Z, const float* __restrict__ base, int64_t rows,
+                                int64_t cols, float* __restrict__ out) {
+  const int64_t n = rows * cols;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = Z[i] + base[i % cols];
+}
+
+// Mean squared error of base + Z against T: dZ = 2 (base + Z - T) / n and
+// per-block partial sums of the squared error (FP64)
+__global__ __launch_bounds__(EW) void mse_kernel(const float* __restrict__ Z, const float* __restrict__ base,
+                                                 const float* __restrict__ T, int64_t rows, int64_t cols,
+                                                 float* __restrict__ dZ, double* __restrict__ partial) {
+  __shared__ double red[EW];
+  const int64_t n = rows * cols;
+  const float g = 2.0f / (float)n;
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)EW + threadIdx.x; i < n; i += (int64_t)gridDim.x * EW) {
+    const float d = Z[i] + base[i % cols] - T[i];
+    dZ[i] = g * d;
+    s += (double)d * (double)d;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = EW / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// G *= 1 - Z^2 (tanh' expressed through the stored activation)
+__global__ void tanh_bwd_kernel(float* __restrict__ G, const float* __restrict__ Z, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    G[i] *= 1.0f - Z[i] * Z[i];
+}
+
+// db[c] = sum_r G[r][c]
+__global__ void col_sum_kernel(const float* __restrict__ G, int64_t rows, int64_t cols, float* __restrict__ db) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < cols; c += (int64_t)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int64_t r = 0; r < rows; ++r) s += G[r * cols + c];
+    db[c] = s;
+  }
+}
+
+// Adam (Kingma & Ba 2015) on FP32 masters; refreshes the BF16 copy
+__global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, uint16_t* __restrict__ ph, int64_t n, float lr, float b1, float b2,
+                            float eps, float c1, float c2) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    if (ph) ph[i] = f2bf(pi);
+  }
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ src, uint16_t* __restrict__ dst, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = f2bf(src[i]);
+}
+
+unsigned grid_for(int64_t n, int block = EW) {
+  int64_t g = (n + block - 1) / block;
+  return (unsigned)(g < 1 ? 1 : g > 65536 ? 65536 : g);
+}
+
+}  // namespace
+
+// ============================================================== host side
+struct so_mlp {
+  so_context* ctx = nullptr;
+  int L = 0;
+  std::vector<int64_t> w;  // widths, L + 1
+  std::vector<float*> W, b, gW, gb, mW, vW, mb, vb;
+  std::vector<uint16_t*> Wh;
+  float* base = nullptr;
+  int64_t cap = 0;
+  std::vector<float*> Z;  // Z[l] for l = 1..L: [cap][w[l]]
+  float* G1 = nullptr;
+  float* G2 = nullptr;
+  float* part = nullptr;
+  size_t part_elems = 0;
+  double* lpart = nullptr;
+  int64_t adam_t = 0;
+  int64_t maxw = 0;
+
+  ~so_mlp() {
+    auto fr = [&](void* p) {
+      if (p) ctx->release(p);
+    };
+    for (int l = 0; l < L; ++l) {
+      fr(W[l]);
+      fr(b[l]);
+      fr(gW[l]);
+      fr(gb[l]);
+      fr(mW[l]);
+      fr(vW[l]);
+      fr(mb[l]);
+      fr(vb[l]);
+      fr(Wh[l]);
+    }
+    fr(base);
+    for (auto* z : Z) fr(z);
+    fr(G1);
+    fr(G2);
+    fr(part);
+    fr(lpart);
+  }
+};
+
+namespace {
+
+void zero_dev(so_context* ctx, void* p, size_t bytes) { HIPCHK(hipMemsetAsync(p, 0, bytes, ctx->stream)); }
+
+void ensure_cap(so_mlp* m, int64_t batch) {
+  if (batch <= m->cap) return;
+  for (auto* z : m->Z)
+    if (z) m->ctx->release(z);
+  m->Z.assign(m->L + 1, nullptr);
+  for (int l = 1; l <= m->L; ++l) m->Z[l] = (float*)m->ctx->alloc(sizeof(float) * batch * m->w[l]);
+  if (m->G1) m->ctx->release(m->G1);
+  if (m->G2) m->ctx->release(m->G2);
+  m->G1 = (float*)m->ctx->alloc(sizeof(float) * batch * m->maxw);
+  m->G2 = (float*)m->ctx->alloc(sizeof(float) * batch * m->maxw);
+  if (m->lpart) m->ctx->release(m->lpart);
+  m->lpart = (double*)m->ctx->alloc(sizeof(double) * 1024);
+  m->cap = batch;
+}
+
+// split count for a GEMM with few output tiles and a long reduction
+int choose_splits(int64_t M, int64_t N, int64_t K) {
+  const int64_t tiles = ((M + GT - 1) / GT) * ((N + GT - 1) / GT);
+  int s = 1;
+  while (tiles * s < 1024 && K / (s * 2) >= 4 * GK && s < 256) s *= 2;
+  return s;
+}
+
+float* ensure_part(so_mlp* m, size_t elems) {
+  if (elems > m->part_elems) {
+    if (m->part) m->ctx->release(m->part);
+    m->part = (float*)m->ctx->alloc(sizeof(float) * elems);
+    m->part_elems = elems;
+  }
+  return m->part;
+}
+
+// C[M][N] = A . B (see gemm_bf16_kernel for the operand forms)
+template <typename TA, typename TB, bool TRA, bool TRB>
+void gemm(so_mlp* m, const TA* A, int64_t lda, const TB* B, int64_t ldb, float* C, int64_t M, int64_t N, int64_t K) {
+  const int splits = choose_splits(M, N, K);
+  int64_t kchunk = (K + splits - 1) / splits;
+  kchunk = (kchunk + GK - 1) / GK * GK;
+  const int s_used = (int)((K + kchunk - 1) / kchunk);
+  dim3 grid((unsigned)((N + GT - 1) / GT), (unsigned)((M + GT - 1) / GT), (unsigned)s_used);
+  if (s_used <= 1) {
+    gemm_bf16_kernel<TA, TB, TRA, TRB><<<grid, GTHREADS, 0, m->ctx->stream>>>(A, lda, B, ldb, C, N, M, N, K, K, 0);
+  } else {
+    float* P = ensure_part(m, (size_t)s_used * M * N);
+    gemm_bf16_kernel<TA, TB, TRA, TRB><<<grid, GTHREADS, 0, m->ctx->stream>>>(A, lda, B, ldb, P, N, M, N, K, kchunk,
+                                                                             M * N);
+    splitk_reduce_kernel<<<grid_for(M * N), EW, 0, m->ctx->stream>>>(P, M * N, s_used, C);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+// forward through all layers; Z[L] holds the raw output (no base)
+void forward(so_mlp* m, const float* x, int64_t batch) {
+  ensure_cap(m, batch);
+  const float* in = x;
+  for (int l = 0; l < m->L; ++l) {
+    const int64_t ni = m->w[l], no = m->w[l + 1];
+    float* Y = m->Z[l + 1];
+    gemm<float, uint16_t, false, true>(m, in, ni, m->Wh[l], ni, Y, batch, no, ni);
+    bias_act_kernel<<<grid_for(batch * no), EW, 0, m->ctx->stream>>>(Y, m->b[l], batch, no, l < m->L - 1 ? 1 : 0, Y);
+    in = Y;
+  }
+  HIPCHK(hipGetLastError());
+}
+
+// gradients of the mean squared error of base + net(x) against t
+double backward(so_mlp* m, const float* x, const float* t, int64_t batch) {
+  forward(m, x, batch);
+  const int L = m->L;
+  const int64_t nout = m->w[L];
+  const unsigned lb = std::min<unsigned>(1024, grid_for(batch * nout));
+  mse_kernel<<<lb, EW, 0, m->ctx->stream>>>(m->Z[L], m->base, t, batch, nout, m->G1, m->lpart);
+  float* G = m->G1;  // dLoss/dZ[l + 1], then dLoss/dY[l]
+  float* H = m->G2;
+  for (int l = L - 1; l >= 0; --l) {
+    const int64_t ni = m->w[l], no = m->w[l + 1];
+    if (l < L - 1) tanh_bwd_kernel<<<grid_for(batch * no), EW, 0, m->ctx->stream>>>(G, m->Z[l + 1], batch * no);
+    const float* in = l == 0 ? x : m->Z[l];
+    // gW[n][k] = sum_m G[m][n] in[m][k]
+    gemm<float, float, true, false>(m, G, no, in, ni, m->gW[l], no, ni, batch);
+    col_sum_kernel<<<grid_for(no), EW, 0, m->ctx->stream>>>(G, batch, no, m->gb[l]);
+    if (l > 0) {
+      // H[m][k] = sum_n G[m][n] W[n][k]
+      gemm<float, uint16_t, false, false>(m, G, no, m->Wh[l], ni, H, batch, ni, no);
+      std::swap(G, H);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  std::vector<double> lp(lb);
+  HIPCHK(hipMemcpyAsync(lp.data(), m->lpart, sizeof(double) * lb, hipMemcpyDeviceToHost, m->ctx->stream));
+  m->ctx->sync();
+  double s = 0.0;
+  for (double v : lp) s += v;
+  return s / (double)(batch * nout);
+}
+
+void adam(so_mlp* m, double lr, double b1, double b2, double eps) {
+  m->adam_t++;
+  const float c1 = (float)(1.0 - std::pow(b1, (double)m->adam_t));
+  const float c2 = (float)(1.0 - std::pow(b2, (double)m->adam_t));
+  for (int l = 0; l < m->L; ++l) {
+    const int64_t nw = m->w[l] * m->w[l + 1], nb = m->w[l + 1];
+    adam_kernel<<<grid_for(nw), EW, 0, m->ctx->stream>>>(m->W[l], m->gW[l], m->mW[l], m->vW[l], m->Wh[l], nw,
+                                                         (float)lr, (float)b1, (float)b2, (float)eps, c1, c2);
+    adam_kernel<<<grid_for(nb), EW, 0, m->ctx->stream>>>(m->b[l], m->gb[l], m->mb[l], m->vb[l], nullptr, nb, (float)lr,
+                                                         (float)b1, (float)b2, (float)eps, c1, c2);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+uint64_t splitmix(uint64_t& s) {
+  s += 0x9E3779B97F4A7C15ull;
+  uint64_t z = s;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+
+extern "C" int so_mlp_create(so_context* ctx, int nlayers, const int64_t* widths, uint64_t seed, so_mlp** out) {
+  API_BEGIN
+  NEED(ctx);
+  NEED(widths);
+  NEED(out);
+  if (nlayers < 1 || nlayers > 16) fail(SO_ERR_INVALID, "nlayers must be in 1..16");
+  for (int l = 0; l <= nlayers; ++l)
+    if (widths[l] < 1 || widths[l] > (1 << 24)) fail(SO_ERR_INVALID, "width %d out of range", l);
+  auto* m = new so_mlp();
+  m->ctx = ctx;
+  m->L = nlayers;
+  m->w.assign(widths, widths + nlayers + 1);
+  for (auto v : m->w) m->maxw = std::max(m->maxw, v);
+  uint64_t s = seed;
+  for (int l = 0; l < nlayers; ++l) {
+    const int64_t ni = m->w[l], no = m->w[l + 1], nw = ni * no;
+    auto mk = [&](int64_t n) { return (float*)ctx->alloc(sizeof(float) * n); };
+    m->W.push_back(mk(nw));
+    m->b.push_back(mk(no));
+    m->gW.push_back(mk(nw));
+    m->gb.push_back(mk(no));
+    m->mW.push_back(mk(nw));
+    m->vW.push_back(mk(nw));
+    m->mb.push_back(mk(no));
+    m->vb.push_back(mk(no));
+    m->Wh.push_back((uint16_t*)ctx->alloc(sizeof(uint16_t) * nw));
+    // Glorot-uniform weights (the output layer scaled down so training starts
+    // from the base prediction), zero biases and moments
+    const double a = std::sqrt(6.0 / (double)(ni + no)) * (l == nlayers - 1 ? 0.1 : 1.0);
+    std::vector<float> hw(nw);
+    for (auto& v : hw) v = (float)(a * (2.0 * ((splitmix(s) >> 11) * (1.0 / 9007199254740992.0)) - 1.0));
+    HIPCHK(hipMemcpy(m->W[l], hw.data(), sizeof(float) * nw, hipMemcpyHostToDevice));
+    for (float* p : {m->b[l], m->mW[l], m->vW[l], m->mb[l], m->vb[l]})
+      zero_dev(ctx, p, sizeof(float) * (p == m->mW[l] || p == m->vW[l] ? nw : no));
+    to_bf16_kernel<<<grid_for(nw), EW, 0, ctx->stream>>>(m->W[l], m->Wh[l], nw);
+  }
+  m->base = (float*)ctx->alloc(sizeof(float) * m->w[nlayers]);
+  zero_dev(ctx, m->base, sizeof(float) * m->w[nlayers]);
+  ctx->sync();
+  *out = m;
+  API_END
+}
+
+extern "C" int so_mlp_destroy(so_mlp* m) {
+  API_BEGIN
+  if (!m) return SO_OK;
+  m->ctx->sync();
+  delete m;
+  API_END
+}
+
+extern "C" int so_mlp_set_base(so_mlp* m, const float* base) {
+  API_BEGIN
+  NEED(m);
+  NEED(base);
+  HIPCHK(hipMemcpy(m->base, base, sizeof(float) * m->w[m->L], hipMemcpyHostToDevice));
+  API_END
+}
+
+extern "C" int so_mlp_params(so_mlp* m, int layer, float* w, float* b) {
+  API_BEGIN
+  NEED(m);
+  if (layer < 0 || layer >= m->L) fail(SO_ERR_INVALID, "layer %d out of range", layer);
+  m->ctx->sync();
+  if (w) HIPCHK(hipMemcpy(w, m->W[layer], sizeof(float) * m->w[layer] * m->w[layer + 1], hipMemcpyDeviceToHost));
+  if (b) HIPCHK(hipMemcpy(b, m->b[layer], sizeof(float) * m->w[layer + 1], hipMemcpyDeviceToHost));
+  API_END
+}
+
+extern "C" int so_mlp_set_params(so_mlp* m, int layer, const float* w, const float* b) {
+  API_BEGIN
+  NEED(m);
+  if (layer < 0 || layer >= m->L) fail(SO_ERR_INVALID, "layer %d out of range", layer);
+  const int64_t nw = m->w[layer] * m->w[layer + 1];
+  m->ctx->sync();
+  if (w) {
+    HIPCHK(hipMemcpy(m->W[layer], w, sizeof(float) * nw, hipMemcpyHostToDevice));
+    to_bf16_kernel<<<grid_for(nw), EW, 0, m->ctx->stream>>>(m->W[layer], m->Wh[layer], nw);
+  }
+  if (b) HIPCHK(hipMemcpy(m->b[layer], b, sizeof(float) * m->w[layer + 1], hipMemcpyHostToDevice));
+  m->ctx->sync();
+  API_END
+}
+
+extern "C" int so_mlp_forward(so_mlp* m, const float* x, int64_t batch, float* y) {
+  API_BEGIN
+  NEED(m);
+  NEED(x);
+  NEED(y);
+  if (batch < 1) fail(SO_ERR_INVALID, "batch must be positive");
+  forward(m, x, batch);
+  const int64_t nout = m->w[m->L];
+  add_base_kernel<<<grid_for(batch * nout), EW, 0, m->ctx->stream>>>(m->Z[m->L], m->base, batch, nout, y);
+  HIPCHK(hipGetLastError());
+  API_END
+}
+
+extern "C" int so_mlp_backward(so_mlp* m, const float* x, const float* target, int64_t batch, double* loss) {
+  API_BEGIN
+  NEED(m);
+  NEED(x);
+  NEED(target);
+  if (batch < 1) fail(SO_ERR_INVALID, "batch must be positive");
+  const double l = backward(m, x, target, batch);
+  if (loss) *loss = l;
+  API_END
+}
+
+extern "C" int so_mlp_gradient(so_mlp* m, int layer, float* gw, float* gb) {
+  API_BEGIN
+  NEED(m);
+  if (layer < 0 || layer >= m->L) fail(SO_ERR_INVALID, "layer %d out of range", layer);
+  m->ctx->sync();
+  if (gw) HIPCHK(hipMemcpy(gw, m->gW[layer], sizeof(float) * m->w[layer] * m->w[layer + 1], hipMemcpyDeviceToHost));
+  if (gb) HIPCHK(hipMemcpy(gb, m->gb[layer], sizeof(float) * m->w[layer + 1], hipMemcpyDeviceToHost));
+  API_END
+}
+
+extern "C" int so_mlp_adam(so_mlp* m, double lr, double beta1, double beta2, double eps) {
+  API_BEGIN
+  NEED(m);
+  adam(m, lr, beta1, beta2, eps);
+  API_END
+}
+
+extern "C" int so_mlp_train_step(so_mlp* m, const float* x, const float* target, int64_t batch, double lr,
+                                 double* loss) {
+  API_BEGIN
+  NEED(m);
+  NEED(x);
+  NEED(target);
+  if (batch < 1) fail(SO_ERR_INVALID, "batch must be positive");
+  const double l = backward(m, x, target, batch);
+  adam(m, lr, 0.9, 0.999, 1e-8);
+  if (loss) *loss = l;
+  API_END
+}
+
+// C[M][N] = op(A) . op(B) for FP32 device operands (BF16 products, FP32
+// sums); A is [M][K] (or [K][M] when tra), B is [K][N] (or [N][K] when trb).
+// Exposes the GEMM for unit tests of its operand layouts.
+extern "C" int so_mlp_gemm(so_mlp* m, const float* A, const float* B, float* C, int64_t M, int64_t N, int64_t K,
+                           int tra, int trb) {
+  API_BEGIN
+  NEED(m);
+  NEED(A);
+  NEED(B);
+  NEED(C);
+  const int64_t lda = tra ? M : K, ldb = trb ? K : N;
+  if (tra && trb)
+    gemm<float, float, true, true>(m, A, lda, B, ldb, C, M, N, K);
+  else if (tra)
+    gemm<float, float, true, false>(m, A, lda, B, ldb, C, M, N, K);
+  else if (trb)
+    gemm<float, float, false, true>(m, A, lda, B, ldb, C, M, N, K);
+  else
+    gemm<float, float, false, false>(m, A, lda, B, ldb, C, M, N, K);
+  m->ctx->sync();
+  API_END
+}

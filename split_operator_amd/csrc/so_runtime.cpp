@@ -37,109 +37,11 @@
 
 #include "../../include/split_operator_hip.h"
 #include "so_device.hpp"
+#include "so_host.hpp"
 
 using sod::MAXD;
 
-// ============================================================== errors
-static thread_local std::string g_err;
-
-static int set_err(int code, const char* fmt, ...) {
-  char buf[1024];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
-  va_end(ap);
-  g_err = buf;
-  return code;
-}
-
-struct SoError {
-  int code;
-  std::string msg;
-};
-
-[[noreturn]] static void fail(int code, const char* fmt, ...) {
-  char buf[1024];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
-  va_end(ap);
-  throw SoError{code, buf};
-}
-
-#define HIPCHK(x)                                                                                   \
-  do {                                                                                              \
-    hipError_t e_ = (x);                                                                            \
-    if (e_ != hipSuccess) fail(SO_ERR_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
-  } while (0)
-
-#define API_BEGIN try {
-#define API_END                                      \
-  }                                                  \
-  catch (const SoError& e) {                         \
-    return set_err(e.code, "%s", e.msg.c_str());     \
-  }                                                  \
-  catch (const std::exception& e) {                  \
-    return set_err(SO_ERR_INVALID, "%s", e.what());  \
-  }                                                  \
-  return SO_OK;
-
-#define NEED(p)                                                            \
-  do {                                                                     \
-    if (!(p)) fail(SO_ERR_INVALID, "null argument: %s", #p);               \
-  } while (0)
-
 extern "C" const char* so_last_error(void) { return g_err.c_str(); }
-
-// ============================================================== context
-struct so_context {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  double* arena = nullptr;  // scalar slots (norms, losses, leak norms, factors)
-  int arena_size = 0;
-  std::vector<int> free_slots;
-  int next_slot = 0;
-  int nonfinite_slot = -1;
-  std::vector<void*> allocations;
-
-  void* alloc(size_t bytes) {
-    void* p = nullptr;
-    HIPCHK(hipMalloc(&p, bytes ? bytes : 16));
-    allocations.push_back(p);
-    return p;
-  }
-  void release(void* p) {
-    if (!p) return;
-    auto it = std::find(allocations.begin(), allocations.end(), p);
-    if (it != allocations.end()) {
-      allocations.erase(it);
-      (void)hipFree(p);
-    }
-  }
-  int slot() {
-    if (!free_slots.empty()) {
-      int s = free_slots.back();
-      free_slots.pop_back();
-      return s;
-    }
-    if (next_slot >= arena_size) fail(SO_ERR_UNSUPPORTED, "scalar arena exhausted (%d slots)", arena_size);
-    return next_slot++;
-  }
-  void free_slot(int s) {
-    if (s >= 0) free_slots.push_back(s);
-  }
-  void set_slot(int s, double v) {
-    HIPCHK(hipMemcpyAsync(arena + s, &v, sizeof(double), hipMemcpyHostToDevice, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-  }
-  double get_slot(int s) {
-    double v = 0;
-    HIPCHK(hipMemcpyAsync(&v, arena + s, sizeof(double), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-    return v;
-  }
-  void sync() { HIPCHK(hipStreamSynchronize(stream)); }
-};
 
 // ============================================================== grids
 struct GridH {

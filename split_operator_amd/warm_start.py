@@ -1,0 +1,282 @@
+"""Learned warm start for imaginary-time ground states of the radial x angular
+problem (no counterpart in the split_operator crate).
+
+A tanh MLP maps the potential parameters (C6, C12, a) of
+V(r, theta) = C12/r^12 - C6/r^6 (1 + a P2(cos theta)) to the ground state on a
+radial x l grid (complex psi(r, l), interleaved re/im). Its training data are
+imaginary-time solutions the solver itself produces (batched, FP32); its
+forward, backward and Adam steps run in the library's own HIP kernels (BF16
+MFMA, FP32 sums). A warm start converts the prediction to the theta
+representation with the Legendre matrices and starts imaginary-time
+propagation there; the plain cold start (Gaussian x P0) stays the fallback.
+
+Device buffers for the network's inputs and targets are torch tensors on the
+library's stream (torch only provides the memory).
+"""
+from __future__ import annotations
+
+import contextlib
+import ctypes as C
+import time
+
+import numpy as np
+
+import split_operator_amd as so
+from split_operator_amd import workloads as W
+from split_operator_amd._lib import call
+
+PARAM_LO = np.array([1.0, 0.5, 0.0])
+PARAM_HI = np.array([3.0, 2.0, 0.3])
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+_stream = None
+
+
+def device_stream():
+    """torch stream object for the library's HIP stream."""
+    global _stream
+    if _stream is None:
+        import torch
+
+        _stream = torch.cuda.ExternalStream(so.stream_handle())
+    return _stream
+
+
+@contextlib.contextmanager
+def on_stream():
+    """Run torch work (allocations, copies) in order with the library's kernels."""
+    import torch
+
+    with torch.cuda.stream(device_stream()):
+        yield
+
+
+def to_device(a):
+    import torch
+
+    with on_stream():
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to("cuda", non_blocking=False)
+
+
+class MLP:
+    """Fully connected tanh network with linear output: y = base + net(x)."""
+
+    def __init__(self, widths, seed=5):
+        self.widths = [int(w) for w in widths]
+        h = C.c_void_p()
+        w = (C.c_int64 * len(self.widths))(*self.widths)
+        call("so_mlp_create", so.context(), len(self.widths) - 1, w, C.c_uint64(seed), C.byref(h))
+        self._h = h
+
+    @property
+    def layers(self):
+        return len(self.widths) - 1
+
+    def set_base(self, base):
+        b = np.ascontiguousarray(base, dtype=np.float32)
+        assert b.size == self.widths[-1]
+        call("so_mlp_set_base", self._h, _fptr(b))
+
+    def params(self, layer):
+        w = np.zeros((self.widths[layer + 1], self.widths[layer]), np.float32)
+        b = np.zeros(self.widths[layer + 1], np.float32)
+        call("so_mlp_params", self._h, layer, _fptr(w), _fptr(b))
+        return w, b
+
+    def set_params(self, layer, w, b):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        assert w.shape == (self.widths[layer + 1], self.widths[layer]) and b.shape == (self.widths[layer + 1],)
+        call("so_mlp_set_params", self._h, layer, _fptr(w), _fptr(b))
+
+    def forward(self, x):
+        import torch
+
+        assert x.is_cuda and x.dtype == torch.float32 and x.is_contiguous() and x.shape[1] == self.widths[0]
+        with on_stream():
+            y = torch.empty((x.shape[0], self.widths[-1]), dtype=torch.float32, device="cuda")
+        call("so_mlp_forward", self._h, C.c_void_p(x.data_ptr()), x.shape[0], C.c_void_p(y.data_ptr()))
+        return y
+
+    def backward(self, x, t):
+        loss = C.c_double()
+        call("so_mlp_backward", self._h, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()), x.shape[0],
+             C.byref(loss))
+        return loss.value
+
+    def gradient(self, layer):
+        gw = np.zeros((self.widths[layer + 1], self.widths[layer]), np.float32)
+        gb = np.zeros(self.widths[layer + 1], np.float32)
+        call("so_mlp_gradient", self._h, layer, _fptr(gw), _fptr(gb))
+        return gw, gb
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        call("so_mlp_adam", self._h, lr, beta1, beta2, eps)
+
+    def train_step(self, x, t, lr):
+        loss = C.c_double()
+        call("so_mlp_train_step", self._h, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()), x.shape[0], lr,
+             C.byref(loss))
+        return loss.value
+
+    def gemm(self, A, B, M, N, K, tra=False, trb=False):
+        """C = op(A) op(B) on the network's BF16 GEMM (unit-test entry)."""
+        import torch
+
+        with on_stream():
+            Cm = torch.empty((M, N), dtype=torch.float32, device="cuda")
+        call("so_mlp_gemm", self._h, C.c_void_p(A.data_ptr()), C.c_void_p(B.data_ptr()), C.c_void_p(Cm.data_ptr()),
+             M, N, K, int(tra), int(trb))
+        return Cm
+
+    def __del__(self):
+        if getattr(self, "_h", None) and so._lib._lib is not None:
+            so._lib._lib.so_mlp_destroy(self._h)
+            self._h = None
+
+
+def normalise_params(p):
+    """(C6, C12, a) -> [-1, 1]^3."""
+    p = np.atleast_2d(np.asarray(p, dtype=np.float64))
+    return (2.0 * (p - PARAM_LO) / (PARAM_HI - PARAM_LO) - 1.0).astype(np.float32)
+
+
+def fix_phase(psi, weights=None):
+    """Unit norm (on the given weights) and a real, positive sum: the global phase of an eigenstate is free."""
+    s = psi.sum()
+    ph = s / abs(s) if abs(s) > 0 else 1.0
+    psi = psi / ph
+    w = 1.0 if weights is None else weights
+    return psi / np.sqrt(np.sum(np.abs(psi) ** 2 * w))
+
+
+class LearnedWarmStart:
+    """Data generation, training and warm-vs-cold evaluation on n_r x n_theta grids (r in [0.5, 50],
+    n_theta Gauss-Legendre nodes, l = 0..n_theta-1), imaginary time step dt."""
+
+    def __init__(self, n_r=1024, n_theta=32, dt=0.05, hidden=(256, 256, 256), seed=5):
+        self.n_r, self.n_theta, self.dt = n_r, n_theta, dt
+        self.gl = so.gauss_legendre(n_theta)
+        theta = np.arccos(np.asarray(self.gl[0]))[::-1].copy()
+        wt = np.asarray(self.gl[1])[::-1].copy()
+        polar = so.Grid.new_custom("polar", theta, wt, 1)
+        self.t, self.tinv = so.legendre_matrices(polar)
+        self.radial = so.Grid.new_linear_continuos("radial", 0.5, 50.0, n_r, 0)
+        self.widths = [3, *hidden, 2 * n_r * n_theta]
+        self.net = MLP(self.widths, seed=seed)
+        self.losses = []
+
+    # psi(r, theta) <-> psi(r, l): lane <- M . lane along theta, M = t (forward) or tinv
+    def to_l(self, psi_theta):
+        return psi_theta @ self.t.T
+
+    def to_theta(self, psi_l):
+        return psi_l @ self.tinv.T
+
+    def _instance(self, params, steps, dtype, psi0=None, leak_frames=0):
+        p, _, _ = W.radial_angular(so, n_r=self.n_r, n_theta=self.n_theta, dt=self.dt, steps=steps, im_time=True,
+                                   border=False, leak=True, dtype=dtype, gl=self.gl, params=tuple(params),
+                                   leak_frames=leak_frames, psi0=psi0)
+        return p
+
+    def generate(self, params, steps, batch=256, dtype="f32", log=None):
+        """Relax each potential from the cold start for `steps` imaginary-time steps (batched); returns
+        the solutions psi(r, l) as rows of interleaved re/im FP32."""
+        out = np.zeros((len(params), 2 * self.n_r * self.n_theta), np.float32)
+        for s in range(0, len(params), batch):
+            props = [self._instance(pp, steps, dtype) for pp in params[s:s + batch]]
+            b = so.Batch()
+            for p in props:
+                b.add(p)
+            b.steps(steps)
+            b.flush()
+            for i, p in enumerate(props):
+                psi_l = fix_phase(self.to_l(p.wave_function().array))
+                out[s + i] = psi_l.astype(np.complex64).view(np.float32).ravel()
+            for p in props:
+                b.remove(p)
+            del b, props
+            if log:
+                log(f"generated {min(s + batch, len(params))}/{len(params)}")
+        return out
+
+    def fit(self, params, targets, epochs=200, minibatch=256, lr=1e-3, lr_final=1e-5, seed=7, log=None):
+        """Adam on mean squared error; the output base is the mean target, the network learns the rest."""
+        base = targets.mean(axis=0)
+        self.net.set_base(base)
+        x = to_device(normalise_params(params))
+        t = to_device(targets)
+        n = len(params)
+        rng = np.random.default_rng(seed)
+        nb = max(1, n // minibatch)
+        total = epochs * nb
+        k = 0
+        import torch
+
+        for ep in range(epochs):
+            perm = torch.from_numpy(rng.permutation(n)[: nb * minibatch].reshape(nb, -1))
+            ep_loss = 0.0
+            for bi in range(nb):
+                with on_stream():
+                    idx = perm[bi].to("cuda")
+                    xb = x.index_select(0, idx).contiguous()
+                    tb = t.index_select(0, idx).contiguous()
+                # cosine decay from lr to lr_final
+                cur = lr_final + 0.5 * (lr - lr_final) * (1.0 + np.cos(np.pi * k / max(total - 1, 1)))
+                ep_loss += self.net.train_step(xb, tb, cur)
+                k += 1
+            self.losses.append(ep_loss / nb)
+            if log and (ep % max(1, epochs // 10) == 0 or ep == epochs - 1):
+                log(f"epoch {ep}: loss {self.losses[-1]:.3e}")
+        return self.losses
+
+    def predict(self, params):
+        """Predicted psi(theta) arrays [n, n_r, n_theta] (complex128) for the given potentials."""
+        x = to_device(normalise_params(params))
+        y = self.net.forward(x)
+        so.synchronize()
+        yl = y.cpu().numpy().astype(np.float32).view(np.complex64).reshape(len(params), self.n_r, self.n_theta)
+        return np.stack([self.to_theta(a.astype(np.complex128)) for a in yl])
+
+    def relax(self, params, horizon, e_ref=None, tol=1e-8, psi0s=None, check_every=256, dtype="f64"):
+        """Batched FP64 imaginary time from the cold start (psi0s None) or the given states. Records the decay
+        energy of every instance at each check; with e_ref, an instance retires once |E - E_ref| <= tol |E_ref|.
+        Returns per-instance dicts and the wall time of the stepping."""
+        insts = []
+        for i, pp in enumerate(params):
+            p = self._instance(pp, horizon, dtype, None if psi0s is None else psi0s[i], leak_frames=2 * horizon)
+            insts.append({"prop": p, "hist": [], "met": None, "steps": 0})
+        b = so.Batch()
+        for it in insts:
+            b.add(it["prop"])
+        active = list(range(len(insts)))
+        so.synchronize()
+        t0 = time.perf_counter()
+        done = 0
+        while active and done < horizon:
+            k = min(check_every, horizon - done)
+            b.steps(k)
+            b.flush()
+            done += k
+            now = time.perf_counter() - t0
+            for i in list(active):
+                it = insts[i]
+                tail = it["prop"].leak.loss_tail(2)
+                if len(tail) < 2:
+                    continue
+                e = W.decay_energy(tail[0], tail[1], self.dt)
+                it["hist"].append((done, now, e))
+                it["steps"] = done
+                if e_ref is not None and abs(e - e_ref[i]) <= tol * abs(e_ref[i]):
+                    it["met"] = (done, now)
+                    active.remove(i)
+                    b.remove(it["prop"])
+        so.synchronize()
+        wall = time.perf_counter() - t0
+        for i in active:
+            b.remove(insts[i]["prop"])
+        return insts, wall

@@ -63,7 +63,7 @@ def radial_angular_params(seed):
 
 def radial_angular(B, n_r=2048, n_theta=128, r_min=0.5, r_max=50.0, dt=0.002, steps=2000, seed=1000,
                    im_time=False, border=True, leak=False, dtype=None, gl=None, r0=8.0, sigma=1.0,
-                   leak_frames=0, params=None):
+                   leak_frames=0, params=None, psi0=None):
     """2-D radial x angular propagation: r in [0.5, 50] (n_r points), theta on n_theta
     Gauss-Legendre nodes, V = C12/r^12 - C6/r^6 (1 + a P2(cos theta)), m = 1, Legendre
     transform to l = 0..n_theta-1 with the centrifugal term l(l+1)/(2 m r^2), radial FFT,
@@ -81,7 +81,8 @@ def radial_angular(B, n_r=2048, n_theta=128, r_min=0.5, r_max=50.0, dt=0.002, st
     polar = B.Grid.new_custom("polar", theta, wt, 1)
     r = radial.nodes
     g_r = np.exp(-(((r - r0) / (2.0 * sigma)) ** 2))
-    psi0 = np.repeat(g_r[:, None], n_theta, axis=1).astype(np.complex128)
+    if psi0 is None:
+        psi0 = np.repeat(g_r[:, None], n_theta, axis=1).astype(np.complex128)
     wf = B.WaveFunction(psi0, [radial, polar]) if dtype is None else B.WaveFunction(psi0, [radial, polar], dtype=dtype)
     wf.normalize(1.0)
     p = B.Propagation()

@@ -1,0 +1,127 @@
+"""The warm-start network's HIP kernels against the f64 CPU oracle (oracle/mlp_oracle.py).
+
+GEMM operand layouts are checked exactly with small-integer operands (exact in BF16, sums exact in
+FP32). Forward outputs, the loss and every gradient are compared with the oracle, which rounds
+GEMM operands to BF16 where the kernels do: they agree to FP32 round-off, except that an FP32 value
+sitting on a BF16 rounding boundary may round the other way, so gradients are compared in relative
+L2 norm (1e-3).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+so = pytest.importorskip("split_operator_amd")
+torch = pytest.importorskip("torch")
+from oracle import mlp_oracle as MO  # noqa: E402
+from split_operator_amd import warm_start as WS  # noqa: E402
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("tra", [False, True])
+@pytest.mark.parametrize("trb", [False, True])
+@pytest.mark.parametrize("mnk", [(70, 45, 100), (64, 64, 8192), (5, 130, 3)])
+def test_gemm_layouts_exact(tra, trb, mnk):
+    M, N, K = mnk
+    rng = np.random.default_rng(M + N + K)
+    A = rng.integers(-8, 9, (M, K)).astype(np.float32)
+    B = rng.integers(-8, 9, (K, N)).astype(np.float32)
+    net = WS.MLP([3, 4, 5])
+    Ad = WS.to_device(A.T.copy() if tra else A)
+    Bd = WS.to_device(B.T.copy() if trb else B)
+    Cm = net.gemm(Ad, Bd, M, N, K, tra, trb)
+    so.synchronize()
+    assert np.array_equal(Cm.cpu().numpy(), (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float32))
+
+
+def _setup(widths, batch, seed=3):
+    rng = np.random.default_rng(seed)
+    net = WS.MLP(widths, seed=11)
+    params = []
+    for l in range(len(widths) - 1):
+        w = rng.normal(0, 1.0 / np.sqrt(widths[l]), (widths[l + 1], widths[l])).astype(np.float32)
+        b = rng.normal(0, 0.1, widths[l + 1]).astype(np.float32)
+        net.set_params(l, w, b)
+        params.append((w.astype(np.float64), b.astype(np.float64)))
+    base = rng.normal(0, 0.2, widths[-1]).astype(np.float32)
+    net.set_base(base)
+    x = rng.uniform(-1, 1, (batch, widths[0])).astype(np.float32)
+    t = rng.normal(0, 0.5, (batch, widths[-1])).astype(np.float32)
+    return net, params, base.astype(np.float64), x, t
+
+
+WIDTHS = [[3, 16, 16, 16, 40], [3, 20, 33, 17, 75], [3, 64, 64, 64, 2048]]
+
+
+@pytest.mark.parametrize("widths", WIDTHS)
+def test_forward_matches_oracle(widths):
+    net, params, base, x, t = _setup(widths, 37)
+    y = net.forward(WS.to_device(x))
+    so.synchronize()
+    ref = MO.forward(params, x.astype(np.float64))[-1] + base
+    assert rel_l2(y.cpu().numpy(), ref) < 1e-5
+
+
+@pytest.mark.parametrize("widths", WIDTHS)
+def test_gradients_match_oracle(widths):
+    net, params, base, x, t = _setup(widths, 37)
+    loss = net.backward(WS.to_device(x), WS.to_device(t))
+    ref_loss, ref = MO.loss_and_grads(params, x.astype(np.float64), t.astype(np.float64), base)
+    assert abs(loss - ref_loss) < 1e-5 * ref_loss
+    for l in range(len(widths) - 1):
+        gw, gb = net.gradient(l)
+        assert rel_l2(gw, ref[l][0]) < 1e-3, l
+        assert rel_l2(gb, ref[l][1]) < 1e-3, l
+
+
+def test_adam_step_matches_oracle():
+    widths = [3, 20, 33, 17, 75]
+    net, params, base, x, t = _setup(widths, 16)
+    net.backward(WS.to_device(x), WS.to_device(t))
+    grads = [net.gradient(l) for l in range(4)]
+    net.adam(1e-2)
+    for l in range(4):
+        w1, b1 = net.params(l)
+        for p0, g, p1 in ((params[l][0], grads[l][0], w1), (params[l][1], grads[l][1], b1)):
+            ref, _, _ = MO.adam_step(p0, g.astype(np.float64), np.zeros_like(p0), np.zeros_like(p0), 1, 1e-2)
+            assert np.max(np.abs(p1 - ref)) < 1e-6
+
+
+def test_training_reduces_loss():
+    """A smooth map [-1, 1]^3 -> R^64 is learned: the loss falls by two orders of magnitude."""
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, (256, 3)).astype(np.float32)
+    k = np.linspace(0, 3, 64)
+    t = (np.sin(x[:, :1] * k + x[:, 1:2]) * (1 + 0.3 * x[:, 2:3])).astype(np.float32)
+    net = WS.MLP([3, 64, 64, 64, 64], seed=2)
+    net.set_base(t.mean(axis=0))
+    xd, td = WS.to_device(x), WS.to_device(t)
+    first = net.train_step(xd, td, 3e-3)
+    for _ in range(600):
+        last = net.train_step(xd, td, 3e-3)
+    assert last < first / 100, (first, last)
+
+
+def test_warm_start_pipeline_small():
+    """End to end at reduced size: solver-generated targets, training on the GPU, prediction for
+    held-out potentials closer to their relaxed states than the cold start."""
+    ws = WS.LearnedWarmStart(n_r=128, n_theta=8, dt=0.05, hidden=(64, 64, 64), seed=3)
+    rng = np.random.default_rng(1)
+    ptrain = np.column_stack([rng.uniform(1, 3, 48), rng.uniform(0.5, 2, 48), rng.uniform(0, 0.3, 48)])
+    ptest = np.column_stack([rng.uniform(1, 3, 4), rng.uniform(0.5, 2, 4), rng.uniform(0, 0.3, 4)])
+    steps = 3000
+    targets = ws.generate(ptrain, steps, batch=16)
+    truth = ws.generate(ptest, steps, batch=4)
+    losses = ws.fit(ptrain, targets, epochs=150, minibatch=16, lr=2e-3)
+    assert losses[-1] < losses[0]
+    pred = ws.predict(ptest)
+    r = ws.radial.nodes
+    cold = np.repeat(np.exp(-(((r - 8.0) / 2.0) ** 2))[:, None], 8, axis=1).astype(np.complex128)
+    for i in range(4):
+        tr = truth[i].view(np.complex64).reshape(128, 8).astype(np.complex128)
+        pl = WS.fix_phase(ws.to_l(pred[i]))
+        cl = WS.fix_phase(ws.to_l(cold))
+        assert np.linalg.norm(pl - tr) < 0.5 * np.linalg.norm(cl - tr)
