@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--c4-max-steps", type=int, default=40000)
     ap.add_argument("--c4-f32-steps", type=int, default=2000, help="steps of the FP32 batched throughput run")
     ap.add_argument("--c4-solo", action="store_true", help="also relax the instances one by one")
+    ap.add_argument("--c5-train", type=int, default=1024, help="training potentials of the warm start (0: skip)")
+    ap.add_argument("--c5-train-steps", type=int, default=20480, help="imaginary-time steps per training solution")
+    ap.add_argument("--c5-eval", type=int, default=64, help="held-out potentials of the warm-start evaluation")
+    ap.add_argument("--c5-epochs", type=int, default=300)
     return ap.parse_args()
 
 
@@ -124,8 +128,8 @@ def ground_states(so, W, count, max_steps, f32_steps, solo):
         rec["solo"] = srec
         rec["solo_max_energy_diff"] = max(abs(a["energy"] - b) for a, b in zip(sgs.inst, energies64))
         del sgs
-    # FP32 storage: batched throughput over a fixed number of steps, and how far its decay
-    # estimator lands from the converged FP64 energies after those steps
+    # FP32 storage: batched throughput over a fixed number of steps, and how far its decay-estimator
+    # energies land from the same steps stored in FP64
     g32 = W.GroundStateBatch(so, count=count, seed0=4000, dt=dt, max_steps=f32_steps, tol=0.0, dtype="f32")
     g32.advance(16)
     so.synchronize()
@@ -134,11 +138,73 @@ def ground_states(so, W, count, max_steps, f32_steps, solo):
     so.synchronize()
     wall = time.perf_counter() - t0
     g32.check()
-    errs = [abs(it["energy"] - e) / abs(e) for it, e in zip(g32.inst, energies64)]
-    rec["f32"] = {"steps": f32_steps - 16, "pt_steps_per_s": g32.points() * (f32_steps - 16) / wall,
+    e32 = [it["energy"] for it in g32.inst]
+    del g32
+    g64 = W.GroundStateBatch(so, count=count, seed0=4000, dt=dt, max_steps=f32_steps, tol=0.0, dtype="f64")
+    g64.advance(f32_steps)
+    g64.check()
+    errs = [abs(a - it["energy"]) / abs(it["energy"]) for a, it in zip(e32, g64.inst)]
+    del g64
+    rec["f32"] = {"steps": f32_steps - 16, "pt_steps_per_s": rec["points"] * (f32_steps - 16) / wall,
                   "ms_per_step": wall / (f32_steps - 16) * 1e3,
-                  "max_rel_energy_err_vs_f64_converged": max(errs)}
+                  "max_rel_energy_diff_vs_f64_same_steps": max(errs)}
     return rec
+
+
+def learned_warm_start(so, args):
+    """Warm start of imaginary time from the network vs the plain cold start, on 1024 x 32 grids
+    (r in [0.5, 50], 32 Gauss-Legendre theta nodes / l = 0..31), dt = 0.05, FP64 propagation with the
+    decay-estimator energy. The reference energy of each held-out potential comes from an untimed cold
+    run of 36864 steps; both timed runs retire an instance once |E - E_ref| <= 1e-8 |E_ref|."""
+    import numpy as np
+
+    from split_operator_amd import warm_start as WS
+
+    def log(msg):
+        print("warm start: " + msg, file=sys.stderr, flush=True)
+
+    ws = WS.LearnedWarmStart(n_r=1024, n_theta=32, dt=0.05, hidden=(256, 256, 256), seed=5)
+    rng = np.random.default_rng(5000)
+    lo, hi = WS.PARAM_LO, WS.PARAM_HI
+    ptrain = lo + (hi - lo) * rng.uniform(size=(args.c5_train, 3))
+    peval = lo + (hi - lo) * np.random.default_rng(6000).uniform(size=(args.c5_eval, 3))
+    t0 = time.perf_counter()
+    targets = ws.generate(ptrain, args.c5_train_steps, batch=256, log=log)
+    so.synchronize()
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    losses = ws.fit(ptrain, targets, epochs=args.c5_epochs, minibatch=256, lr=1e-3, lr_final=1e-5, log=log)
+    so.synchronize()
+    t_fit = time.perf_counter() - t0
+    del targets
+    log("reference run")
+    ref, _ = ws.relax(peval, 36864)
+    e_ref = [it["hist"][-1][2] for it in ref]
+    del ref
+    log("cold run")
+    cold, t_cold = ws.relax(peval, 36864, e_ref=e_ref)
+    t0 = time.perf_counter()
+    psi0 = ws.predict(peval)
+    t_inf = time.perf_counter() - t0
+    log("warm run")
+    warm, t_warm = ws.relax(peval, 36864, e_ref=e_ref, psi0s=psi0)
+    met = lambda runs: [it["met"] for it in runs]  # noqa: E731
+    ok_c, ok_w = all(m is not None for m in met(cold)), all(m is not None for m in met(warm))
+    err_w = max(abs(it["hist"][-1][2] - e) / abs(e) for it, e in zip(warm, e_ref))
+    return {
+        "grid": [1024, 32], "dt": 0.05, "storage": "complex f64 propagation, FP32 training solutions",
+        "network": {"widths": ws.widths, "weights": "bf16 (fp32 Adam masters)", "accumulate": "fp32"},
+        "train": {"potentials": args.c5_train, "steps_per_solution": args.c5_train_steps,
+                  "generate_s": t_gen, "epochs": args.c5_epochs, "fit_s": t_fit, "loss_first": losses[0],
+                  "loss_last": losses[-1]},
+        "eval_potentials": args.c5_eval, "criterion": "|E - E_ref| <= 1e-8 |E_ref|, E_ref from 36864 cold steps",
+        "cold": {"time_s": t_cold, "all_met": ok_c, "steps_max": max(it["steps"] for it in cold),
+                 "steps_mean": float(np.mean([it["steps"] for it in cold]))},
+        "warm": {"time_s": t_warm + t_inf, "inference_s": t_inf, "all_met": ok_w,
+                 "steps_max": max(it["steps"] for it in warm),
+                 "steps_mean": float(np.mean([it["steps"] for it in warm])), "max_rel_energy_err": err_w},
+        "speedup": t_cold / (t_warm + t_inf),
+    }
 
 
 def main():
@@ -213,6 +279,8 @@ def main():
     if args.c4_instances > 0:
         extra["ground_states_batched"] = ground_states(so, W, args.c4_instances, args.c4_max_steps,
                                                        args.c4_f32_steps, args.c4_solo)
+    if args.c5_train > 0:
+        extra["learned_warm_start"] = learned_warm_start(so, args)
 
     line = {
         "metric": METRIC, "value": value, "unit": "grid-point-steps/s", "n_gpus": 1, "steps": args.steps,

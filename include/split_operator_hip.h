@@ -228,10 +228,11 @@ typedef struct so_mlp so_mlp;
 int so_mlp_create(so_context *ctx, int nlayers, const int64_t *widths, uint64_t seed, so_mlp **out);
 int so_mlp_destroy(so_mlp *m);
 int so_mlp_set_base(so_mlp *m, const float *base);                              /* host, widths[nlayers] values */
+int so_mlp_set_output_scale(so_mlp *m, float scale);                            /* y = base + scale net(x), default 1 */
 int so_mlp_params(so_mlp *m, int layer, float *w, float *b);                    /* read FP32 masters (host) */
 int so_mlp_set_params(so_mlp *m, int layer, const float *w, const float *b);
-int so_mlp_forward(so_mlp *m, const float *x, int64_t batch, float *y);         /* y = base + net(x) */
-/* gradients of mean((base + net(x) - target)^2) into the network's gradient buffers; *loss = that mean */
+int so_mlp_forward(so_mlp *m, const float *x, int64_t batch, float *y);         /* y = base + scale net(x) */
+/* gradients of mean((base + scale net(x) - target)^2) into the network's gradient buffers; *loss = that mean */
 int so_mlp_backward(so_mlp *m, const float *x, const float *target, int64_t batch, double *loss);
 int so_mlp_gradient(so_mlp *m, int layer, float *gw, float *gb);                /* read gradients (host) */
 int so_mlp_adam(so_mlp *m, double lr, double beta1, double beta2, double eps);  /* one Adam step on the gradients */

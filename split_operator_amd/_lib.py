@@ -107,6 +107,7 @@ SIGNATURES = {
     "so_mlp_create": [_vp, C.c_int, _i64p, C.c_uint64, _vpp],
     "so_mlp_destroy": [_vp],
     "so_mlp_set_base": [_vp, _fp],
+    "so_mlp_set_output_scale": [_vp, C.c_float],
     "so_mlp_params": [_vp, C.c_int, _fp, _fp],
     "so_mlp_set_params": [_vp, C.c_int, _fp, _fp],
     "so_mlp_forward": [_vp, _vp, C.c_int64, _vp],

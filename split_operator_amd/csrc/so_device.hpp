@@ -23,6 +23,8 @@ constexpr int MAX_TL = 64;      // lanes per workgroup tile
 constexpr int NTHREADS = 256;   // threads per workgroup (4 wave64)
 constexpr int MAX_LDS_ARR = 4;  // small per-pass arrays staged in LDS at kernel start
 constexpr int MAX_PF = 2;       // full-grid tables prefetched into registers at kernel start
+constexpr int MAX_LF = 4;       // 1-D tables off the lane axis, read once per lane at kernel start
+constexpr int MAX_HS = 4;       // scalar slots (EK_SCALE factors) read once at kernel start
 constexpr int FFT_EPT = 16;     // max complex elements per thread of a transform tile (tl * n <= FFT_EPT * NTHREADS)
 #define SOD_MAX_DYN_LDS (152 * 1024)  // dynamic LDS bytes a pass may request per workgroup (gfx950: 160 KiB)
 #define SOD_PREF_DYN_LDS (64 * 1024)  // tiles are sized to this when a single lane fits, two workgroups per CU
@@ -45,6 +47,8 @@ struct ElemOp {
   const double* w_axis;  // EK_NORM: weights of the pass axis for the grid current at that point
   int32_t lds;    // >= 0: the table (EK_MUL1D on the pass axis) or weights (EK_NORM) are staged in LDS array `lds`
   int32_t pf;     // >= 0: EK_MULND values prefetched into register set `pf` at kernel start
+  int32_t lf;     // >= 0: EK_MUL1D off the lane axis: per-lane factor set `lf`; EK_SCALE: hoisted scalar `lf`
+  int32_t pad;
 };
 
 struct Stage {
@@ -126,7 +130,16 @@ struct PassDesc {
   int32_t n_pf;
   int32_t pad5;
   const void* pf_src[MAX_PF];
-  // scalar program inherited from the previous pass (run by workgroup 0 at start)
+  // 1-D tables indexed by an axis other than the lane axis: one factor per lane, read at kernel start
+  int32_t n_lf;
+  int32_t n_hs;
+  const void* lf_src[MAX_LF];
+  int32_t lf_axis[MAX_LF];
+  int32_t hs_slot[MAX_HS];  // scalar slots the EK_SCALE ops read (final before the launch)
+  // scalar program inherited from the previous pass (run by workgroup 0 at
+  // start, or by one extra workgroup when inherit_block is set)
+  int32_t inherit_block;
+  int32_t pad8;
   int32_t prev_nscalar;
   int32_t pad6;
   const double* prev_partials;
@@ -146,7 +159,7 @@ extern "C++" {
 int sod_launch_pass(int dtype, const sod::PassDesc* d_desc, const sod::PassDesc& h_desc, hipStream_t s);
 int sod_launch_scalar(const sod::PassDesc* d_desc, hipStream_t s);
 int sod_pass_lds_bytes(int dtype, const sod::PassDesc& h);
-int sod_launch_batch(int dtype, int nx, int xf_type, const sod::PassDesc* const* d_descs, const int64_t* d_starts,
+int sod_launch_batch(int dtype, int nx, int xf_type, const sod::PassDesc* d_descs, const int64_t* d_starts,
                      int ninst, int64_t total_tiles, int ept, size_t lds, hipStream_t s);
 int sod_launch_state_density(int dtype, const void* psi, int ndim, const int64_t* shape, int axis,
                              const double* const* w, double* out, hipStream_t s);

@@ -422,6 +422,57 @@ __device__ __forceinline__ void mat_store_tile(T* Cm, const ACC& acc, int rt, in
   }
 }
 
+// Register-B product for NCH chunks of 8 k-steps (kpad <= 32 NCH): every k-step
+// of a chunk runs unconditionally (A and B are zero past kdim / kpad), so the
+// MFMA chains are straight-line code with the accumulators kept in place; A
+// is read at a clamped index and zeroed by a select rather than a branch.
+template <typename T, typename T2, int NCH>
+__device__ __forceinline__ void mat_fast_run(const T* A, T* Cm, int tl, int ld, const XformDesc& x, int tid,
+                                             BRegs<T>& br) {
+  using M = Mfma<T>;
+  constexpr int TPW = BFast<T>::TPW, AC = 8;
+  static_assert(NCH * AC <= BFast<T>::KS, "chunks exceed the B fragment");
+  const int rows = x.cplx ? tl : 2 * tl;
+  const int rtiles = (rows + 15) / 16;
+  const int ctiles = x.npad / 16;
+  const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const int kdim = x.kdim, ncols = x.ncols;
+  const MatPlan pl = mat_plan(ctiles, wave);
+  const int astride = x.cplx ? 1 : 2;
+  for (int v0 = 0; pl.col0 + pl.cstep * v0 < ctiles; v0 += TPW) {
+    if (v0 > 0) mat_load_b<T>(x, tid, v0, br);
+    int ct[TPW];
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) ct[u] = pl.col0 + pl.cstep * (v0 + u);
+    for (int rt = pl.row0; rt < rtiles; rt += pl.rstep) {
+      const int row = rt * 16 + lr;
+      const bool rv = row < rows;
+      const int rc = rv ? row : 0;
+      const int abase = x.cplx ? rc * ld * 2 : (rc >> 1) * ld * 2 + (rc & 1);
+      typename M::acc_t acc[TPW];
+#pragma unroll
+      for (int u = 0; u < TPW; ++u) acc[u] = typename M::acc_t{0, 0, 0, 0};
+#pragma unroll
+      for (int c0 = 0; c0 < NCH * AC; c0 += AC) {
+        T av[AC];
+#pragma unroll
+        for (int c = 0; c < AC; ++c) {
+          const int k = 4 * (c0 + c) + lk;
+          const T v = lld(&A[abase + (k < kdim ? k : 0) * astride]);
+          av[c] = (rv && k < kdim) ? v : (T)0;
+        }
+#pragma unroll
+        for (int c = 0; c < AC; ++c)
+#pragma unroll
+          for (int u = 0; u < TPW; ++u) acc[u] = M::run(av[c], br.b[u][c0 + c], acc[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < TPW; ++u)
+        if (ct[u] < ctiles) mat_store_tile<T, T2>(Cm, acc[u], rt, ct[u], rows, ncols, ld, x.cplx, lr, lk);
+    }
+  }
+}
+
 // out-lanes = M . in-lanes for `tl` lanes in LDS. Real matrix: GEMM rows are
 // (lane, re|im) pairs, K = n, columns = n. Complex matrix: rows are lanes seen
 // as 2n reals, K = 2n, columns = 2n, B holds the real 2n x 2n form of M^T.
@@ -443,48 +494,16 @@ __device__ __forceinline__ void mat_lds(const T2* in, T2* out, int tl, int ld, c
   const int lk = lane >> 4;
   const int kdim = x.kdim, kpad = x.kpad, npad = x.npad, ncols = x.ncols;
   if (fast) {
-    constexpr int TPW = BFast<T>::TPW, KS = BFast<T>::KS;
-    const MatPlan pl = mat_plan(ctiles, wave);
-    const int astride = x.cplx ? 1 : 2;
-    for (int v0 = 0; pl.col0 + pl.cstep * v0 < ctiles; v0 += TPW) {
-      if (v0 > 0) mat_load_b<T>(x, tid, v0, br);
-      int ct[TPW];
-      bool has[TPW];
-#pragma unroll
-      for (int u = 0; u < TPW; ++u) {
-        ct[u] = pl.col0 + pl.cstep * (v0 + u);
-        has[u] = ct[u] < ctiles;
-      }
-      for (int rt = pl.row0; rt < rtiles; rt += pl.rstep) {
-        const int row = rt * 16 + lr;
-        const bool rv = row < rows;
-        const int abase = x.cplx ? row * ld * 2 : (row >> 1) * ld * 2 + (row & 1);
-        typename M::acc_t acc[TPW];
-#pragma unroll
-        for (int u = 0; u < TPW; ++u) acc[u] = typename M::acc_t{0, 0, 0, 0};
-        // A fragments in chunks of AC k-steps keep few registers live
-        constexpr int AC = 8;
-#pragma unroll
-        for (int c0 = 0; c0 < KS; c0 += AC)
-          if (4 * c0 < kpad) {
-            T av[AC];
-#pragma unroll
-            for (int c = 0; c < AC; ++c) {
-              const int k = 4 * (c0 + c) + lk;
-              av[c] = (rv && k < kdim) ? lld(&A[abase + k * astride]) : (T)0;
-            }
-#pragma unroll
-            for (int c = 0; c < AC; ++c)
-              if (4 * (c0 + c) < kpad) {
-#pragma unroll
-                for (int u = 0; u < TPW; ++u) acc[u] = M::run(av[c], br.b[u][c0 + c], acc[u]);
-              }
-          }
-#pragma unroll
-        for (int u = 0; u < TPW; ++u)
-          if (has[u]) mat_store_tile<T, T2>(Cm, acc[u], rt, ct[u], rows, ncols, ld, x.cplx, lr, lk);
-      }
-    }
+    // chunks of AC k-steps that cover kpad: straight-line MFMA chains
+    const int nch = (kpad / 4 + 7) / 8;
+    if (nch <= 1)
+      mat_fast_run<T, T2, 1>(A, Cm, tl, ld, x, tid, br);
+    else if (nch == 2)
+      mat_fast_run<T, T2, 2>(A, Cm, tl, ld, x, tid, br);
+    else if (nch == 3)
+      mat_fast_run<T, T2, 3>(A, Cm, tl, ld, x, tid, br);
+    else
+      mat_fast_run<T, T2, 4>(A, Cm, tl, ld, x, tid, br);
     return;
   }
   constexpr int KC = KChunk<T>::value;
@@ -520,9 +539,14 @@ struct LaneTab {
   double w[MAX_TL];
   int32_t idx[MAX_TL][MAXD];
   int32_t valid[MAX_TL];
+  double lf[MAX_LF][MAX_TL][2];  // per-lane factors of 1-D tables off the lane axis
+  double hs[MAX_HS];             // EK_SCALE factors
 };
 
-__device__ void setup_lanes(const PassDesc* __restrict__ d, LaneTab& lt, int64_t tile, int tid) {
+template <typename T>
+__device__ __forceinline__ void setup_lanes(const PassDesc* __restrict__ d, LaneTab& lt, int64_t tile, int tid) {
+  using T2 = typename CT<T>::T2;
+  if (tid >= 64 && tid < 64 + d->n_hs) lt.hs[tid - 64] = gld(&d->S[d->hs_slot[tid - 64]]);
   const int tl = d->tl;
   if (tid < tl) {
     const int l = tid;
@@ -562,6 +586,11 @@ __device__ void setup_lanes(const PassDesc* __restrict__ d, LaneTab& lt, int64_t
         if (b != axis) w *= gld(&d->w_other[b][lt.idx[l][b]]);
     }
     lt.w[l] = w;
+    for (int j = 0; j < d->n_lf; ++j) {
+      const T2 f = gld(&((const T2*)d->lf_src[j])[lt.idx[l][d->lf_axis[j]]]);
+      lt.lf[j][l][0] = (double)f.x;
+      lt.lf[j][l][1] = (double)f.y;
+    }
   }
 }
 
@@ -600,6 +629,15 @@ __device__ __forceinline__ void stage_vec(const Stage& st, T2 (&z)[EPT], const i
 #pragma unroll
         for (int q = 0; q < EPT; ++q)
           if (tid + q * NTHREADS < tot) z[q] = cmul(z[q], lld(&tab[qk[q]]));
+      } else if (e.lf >= 0) {
+#pragma unroll
+        for (int q = 0; q < EPT; ++q)
+          if (tid + q * NTHREADS < tot) {
+            T2 f;
+            f.x = (T)lt.lf[e.lf][ql[q]][0];
+            f.y = (T)lt.lf[e.lf][ql[q]][1];
+            z[q] = cmul(z[q], f);
+          }
       } else {
         const T2* tab = (const T2*)e.table;
 #pragma unroll
@@ -628,7 +666,7 @@ __device__ __forceinline__ void stage_vec(const Stage& st, T2 (&z)[EPT], const i
 #pragma unroll
       for (int q = 0; q < EPT; ++q) z[q] = cmul(z[q], c);
     } else if (kind == EK_SCALE) {
-      const T sc = (T)gld(&d->S[e.slot]);
+      const T sc = e.lf >= 0 ? (T)lt.hs[e.lf] : (T)gld(&d->S[e.slot]);
 #pragma unroll
       for (int q = 0; q < EPT; ++q) {
         z[q].x *= sc;
@@ -654,33 +692,44 @@ __device__ __forceinline__ void stage_vec(const Stage& st, T2 (&z)[EPT], const i
   }
 }
 
-__device__ void run_program(const ScalarOp* sc, int ns, double* S, const double* acc, int nonfinite_slot) {
+// Scalar slots are read and written with device-coherent accesses: a program
+// may run in another workgroup (on another XCD) than the one that ran the
+// previous program of the same launch, with no fence in between.
+__device__ __forceinline__ double sld(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void sst(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void run_program(const ScalarOp* sc, int ns, double* S, const double* acc, int nonfinite_slot) {
   for (int i = 0; i < ns; ++i) {
     const ScalarOp& s = sc[i];
     switch (s.kind) {
       case SK_SET_ACC:
-        S[s.a] = acc[s.b];
-        if (!isfinite(acc[s.b]) && nonfinite_slot >= 0) S[nonfinite_slot] = 1.0;
+        sst(&S[s.a], acc[s.b]);
+        if (!isfinite(acc[s.b]) && nonfinite_slot >= 0) sst(&S[nonfinite_slot], 1.0);
         break;
-      case SK_COPY: S[s.a] = S[s.b]; break;
+      case SK_COPY: sst(&S[s.a], sld(&S[s.b])); break;
       case SK_LOSS: {
-        S[s.a] = S[s.a] + (S[s.b] - S[s.c]);
+        const double loss = sld(&S[s.a]) + (sld(&S[s.b]) - sld(&S[s.c]));
+        sst(&S[s.a], loss);
         if (s.hist) {
-          const double cnt = S[s.cnt_slot];
+          const double cnt = sld(&S[s.cnt_slot]);
           const int64_t c = (int64_t)cnt;
-          if (c % s.freq == 0 && c / s.freq < s.frames) s.hist[c / s.freq] = S[s.a];
-          S[s.cnt_slot] = cnt + 1.0;
+          if (c % s.freq == 0 && c / s.freq < s.frames) sst(&s.hist[c / s.freq], loss);
+          sst(&S[s.cnt_slot], cnt + 1.0);
         }
         break;
       }
-      case SK_SCALE: S[s.a] = sqrt(S[s.b] / S[s.c]); break;
-      case SK_CONST: S[s.a] = s.v; break;
+      case SK_SCALE: sst(&S[s.a], sqrt(sld(&S[s.b]) / sld(&S[s.c]))); break;
+      case SK_CONST: sst(&S[s.a], s.v); break;
       default: break;
     }
   }
 }
 
-__device__ void run_scalar_program(const PassDesc* __restrict__ d, const double* acc) {
+__device__ __forceinline__ void run_scalar_program(const PassDesc* __restrict__ d, const double* acc) {
   run_program(d->sc, d->nscalar, d->S, acc, d->nonfinite_slot);
 }
 
@@ -691,7 +740,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Sums per-tile partials in a fixed order (strided per thread, then wave and
 // block trees); every thread of the block gets the four totals.
-__device__ void sum_partials(const double* partials, int64_t nt, int tid, double* out, bool coherent) {
+__device__ __forceinline__ void sum_partials(const double* partials, int64_t nt, int tid, double* out, bool coherent) {
   __shared__ double red2[NTHREADS / 64][MAX_ACC];
   const int wave = tid >> 6, lane = tid & 63;
   double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
@@ -728,10 +777,11 @@ __device__ void sum_partials(const double* partials, int64_t nt, int tid, double
   }
 }
 
-// Workgroup 0 runs the scalar program the previous pass deferred (its norm
-// partials are visible here because a kernel boundary separates the passes).
-__device__ void run_inherited(const PassDesc* __restrict__ d, int tid, int64_t tile) {
-  if (d->prev_nscalar == 0 || tile != 0) return;
+// Runs the scalar program the previous pass deferred (its norm partials are
+// visible here because a kernel boundary separates the passes). Called by one
+// workgroup: workgroup 0 of the pass, or an extra workgroup of its own.
+__device__ __forceinline__ void run_inherited(const PassDesc* __restrict__ d, int tid) {
+  if (d->prev_nscalar == 0) return;
   double acc[MAX_ACC];
   sum_partials(d->prev_partials, d->prev_ntiles, tid, acc, false);
   if (tid == 0) run_program(d->prev_sc, d->prev_nscalar, d->S, acc, d->nonfinite_slot);
@@ -746,7 +796,7 @@ __device__ void run_inherited(const PassDesc* __restrict__ d, int tid, int64_t t
 // and read with device-coherent (agent-scope) atomic accesses, and each
 // workgroup waits for its partial stores to complete before it bumps the
 // arrival counter, so the last arrival reads complete partials.
-__device__ void finish_pass(const PassDesc* __restrict__ d, const Acc& in, int tid, int64_t tile, int64_t ntiles) {
+__device__ __forceinline__ void finish_pass(const PassDesc* __restrict__ d, const Acc& in, int tid, int64_t tile, int64_t ntiles) {
   __shared__ double red[NTHREADS / 64][MAX_ACC];
   __shared__ int is_last;
   if (!d->has_norm) {
@@ -804,14 +854,16 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t total) {
   return x * per + (x < rem ? x : rem) + j;
 }
 
+
 // ---------------------------------------------------------------- the pass kernels
 // Element-wise pass (no transform): a flat, coalesced sweep; each workgroup
 // owns NTHREADS * EPT consecutive elements.
 template <typename T, int EPT>
-__device__ __forceinline__ void elem_pass_body(const PassDesc* __restrict__ d, int64_t tile, int64_t ntiles) {
+__device__ __forceinline__ void elem_pass_body(const PassDesc* __restrict__ d, int64_t tile, int64_t ntiles,
+                                               bool inherit) {
   using T2 = typename CT<T>::T2;
   const int tid = threadIdx.x;
-  run_inherited(d, tid, tile);
+  if (inherit) run_inherited(d, tid);
   T2* psi = (T2*)d->psi;
   const int nd = d->ndim;
   int64_t stride[MAXD];
@@ -886,9 +938,17 @@ __device__ __forceinline__ void elem_pass_body(const PassDesc* __restrict__ d, i
   finish_pass(d, acc, tid, tile, ntiles);
 }
 
+// With inherit_block, one workgroup beyond the ntiles tiles runs the previous
+// pass's deferred program, off the critical path of the tiles.
 template <typename T, int EPT>
-__global__ __launch_bounds__(NTHREADS) void elem_pass_kernel(const PassDesc* __restrict__ d) {
-  elem_pass_body<T, EPT>(d, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
+__global__ __launch_bounds__(NTHREADS) void elem_pass_kernel(const PassDesc* __restrict__ d, int64_t ntiles,
+                                                             int inherit_block) {
+  const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
+  if (b >= ntiles) {
+    run_inherited(d, threadIdx.x);
+    return;
+  }
+  elem_pass_body<T, EPT>(d, b, ntiles, b == 0 && !inherit_block);
 }
 
 // LDS position of element k of a lane: padded in FFT passes, plain in matrix passes
@@ -928,7 +988,8 @@ __device__ __forceinline__ void map_global(int (&ql)[EPT], int (&qk)[EPT], int t
 // (psi, full-grid tables of later stages, 1-D tables and norm weights, GEMM B)
 // is issued at kernel start, so the memory latency is paid once.
 template <typename T, int KIND, int EPT>
-__device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, int64_t tile, int64_t ntiles) {
+__device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, int64_t tile, int64_t ntiles,
+                                                bool inherit) {
   using T2 = typename CT<T>::T2;
   constexpr int PFN = EPT <= 8 ? MAX_PF : 0;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -942,8 +1003,38 @@ __device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, 
   T2* cur = (T2*)smem;
   T2* oth = cur + tl * ld;
   const T2* twl = (const T2*)(smem + d->lds_arr_off[0]);
-  // stage small arrays (twiddles, 1-D tables, weights) into LDS: all loads of
-  // a chunk are issued before any store, so the latency is paid once
+  BRegs<T> br;
+  if constexpr (KIND == XF_MAT) {
+    if (mat_fast<T>(d->xf[0], tl)) mat_prefetch<T>(d->xf[0], tl, tid, br);
+  }
+  setup_lanes<T>(d, lt, tile, tid);
+  __syncthreads();
+  T2* psi = (T2*)d->psi;
+  const bool strided = inner > 1;
+  int ql[EPT], qk[EPT];
+  map_global<EPT>(ql, qk, tid, tot, tl, n, strided);
+  T2 z[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    z[q].x = 0;
+    z[q].y = 0;
+    if (tid + q * NTHREADS < tot && lt.valid[ql[q]]) z[q] = gld(&psi[lt.base[ql[q]] + (int64_t)qk[q] * inner]);
+  }
+  // full-grid tables of the later stages, into registers
+  T2 pf[PFN > 0 ? PFN : 1][EPT];
+  if constexpr (PFN > 0) {
+#pragma unroll
+    for (int u = 0; u < PFN; ++u)
+      if (u < d->n_pf) {
+        const T2* tab = (const T2*)d->pf_src[u];
+#pragma unroll
+        for (int q = 0; q < EPT; ++q)
+          if (tid + q * NTHREADS < tot) pf[u][q] = gld(&tab[lt.base[ql[q]] + (int64_t)qk[q] * inner]);
+      }
+  }
+  // stage small arrays (twiddles, 1-D tables, weights) into LDS while the psi
+  // loads above are in flight: all loads of a chunk are issued before any
+  // store, so the latency is paid once
   {
     const int na = d->n_lds_arr;
     int maxw = 0;
@@ -975,36 +1066,8 @@ __device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, 
         }
     }
   }
-  BRegs<T> br;
-  if constexpr (KIND == XF_MAT) {
-    if (mat_fast<T>(d->xf[0], tl)) mat_prefetch<T>(d->xf[0], tl, tid, br);
-  }
-  setup_lanes(d, lt, tile, tid);
   __syncthreads();
-  T2* psi = (T2*)d->psi;
-  const bool strided = inner > 1;
-  int ql[EPT], qk[EPT];
-  map_global<EPT>(ql, qk, tid, tot, tl, n, strided);
-  T2 z[EPT];
-#pragma unroll
-  for (int q = 0; q < EPT; ++q) {
-    z[q].x = 0;
-    z[q].y = 0;
-    if (tid + q * NTHREADS < tot && lt.valid[ql[q]]) z[q] = gld(&psi[lt.base[ql[q]] + (int64_t)qk[q] * inner]);
-  }
-  // full-grid tables of the later stages, into registers
-  T2 pf[PFN > 0 ? PFN : 1][EPT];
-  if constexpr (PFN > 0) {
-#pragma unroll
-    for (int u = 0; u < PFN; ++u)
-      if (u < d->n_pf) {
-        const T2* tab = (const T2*)d->pf_src[u];
-#pragma unroll
-        for (int q = 0; q < EPT; ++q)
-          if (tid + q * NTHREADS < tot) pf[u][q] = gld(&tab[lt.base[ql[q]] + (int64_t)qk[q] * inner]);
-      }
-  }
-  run_inherited(d, tid, tile);
+  if (inherit) run_inherited(d, tid);
   Acc acc;
   stage_vec<T, EPT, PFN>(d->st[0], z, ql, qk, tid, tot, lt, d, acc, smem, pf);
 #pragma unroll
@@ -1027,9 +1090,11 @@ __device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, 
     if constexpr (KIND == XF_FFT) {
       fft_lds<EPT, T>(cur, tl, n, ld, twl, x.scale, x.inverse != 0, x.nrad, x.rad, tid);
     } else {
+      // B of the first product was loaded at kernel start; B of the second
+      // is loaded right after the first product, ahead of the stage between
       const bool fx = mat_fast<T>(x, tl);
-      if (xi == 1 && fx) mat_prefetch<T>(x, tl, tid, br);
       mat_lds<T>(cur, oth, tl, ld, x, tid, br, fx);
+      if (xi == 0 && nx == 2 && mat_fast<T>(d->xf[1], tl)) mat_prefetch<T>(d->xf[1], tl, tid, br);
       __syncthreads();
       T2* t = cur;
       cur = oth;
@@ -1047,8 +1112,14 @@ __device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, 
 }
 
 template <typename T, int KIND, int EPT>
-__global__ __launch_bounds__(NTHREADS) void xform_pass_kernel(const PassDesc* __restrict__ d) {
-  xform_pass_body<T, KIND, EPT>(d, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
+__global__ __launch_bounds__(NTHREADS) void xform_pass_kernel(const PassDesc* __restrict__ d, int64_t ntiles,
+                                                              int inherit_block) {
+  const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
+  if (b >= ntiles) {
+    run_inherited(d, threadIdx.x);
+    return;
+  }
+  xform_pass_body<T, KIND, EPT>(d, b, ntiles, b == 0 && !inherit_block);
 }
 
 // One launch for the same pass position of many independent wave functions
@@ -1058,7 +1129,7 @@ __device__ __forceinline__ int batch_instance(const int64_t* starts, int ninst, 
   int lo = 0, hi = ninst;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
-    if (gld(&starts[mid]) <= b)
+    if (starts[mid] <= b)
       lo = mid;
     else
       hi = mid;
@@ -1066,26 +1137,28 @@ __device__ __forceinline__ int batch_instance(const int64_t* starts, int ninst, 
   return lo;
 }
 
+// The members' descriptors are stored contiguously and indexed from the
+// kernel-argument pointer, so their fields are read with scalar loads.
 template <typename T, int KIND, int EPT>
-__global__ __launch_bounds__(NTHREADS) void xform_batch_kernel(const PassDesc* const* __restrict__ ds,
+__global__ __launch_bounds__(NTHREADS) void xform_batch_kernel(const PassDesc* __restrict__ ds,
                                                                const int64_t* __restrict__ starts, int ninst) {
   const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
   const int i = batch_instance(starts, ninst, b);
-  const int64_t s0 = gld(&starts[i]);
-  xform_pass_body<T, KIND, EPT>(gld(&ds[i]), b - s0, gld(&starts[i + 1]) - s0);
+  const int64_t s0 = starts[i];
+  xform_pass_body<T, KIND, EPT>(ds + i, b - s0, starts[i + 1] - s0, b == s0);
 }
 
 template <typename T, int EPT>
-__global__ __launch_bounds__(NTHREADS) void elem_batch_kernel(const PassDesc* const* __restrict__ ds,
+__global__ __launch_bounds__(NTHREADS) void elem_batch_kernel(const PassDesc* __restrict__ ds,
                                                               const int64_t* __restrict__ starts, int ninst) {
   const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
   const int i = batch_instance(starts, ninst, b);
-  const int64_t s0 = gld(&starts[i]);
-  elem_pass_body<T, EPT>(gld(&ds[i]), b - s0, gld(&starts[i + 1]) - s0);
+  const int64_t s0 = starts[i];
+  elem_pass_body<T, EPT>(ds + i, b - s0, starts[i + 1] - s0, b == s0);
 }
 
 __global__ __launch_bounds__(NTHREADS) void scalar_kernel(const PassDesc* __restrict__ d) {
-  run_inherited(d, threadIdx.x, 0);
+  run_inherited(d, threadIdx.x);
   __syncthreads();
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     double acc[MAX_ACC] = {0, 0, 0, 0};
@@ -1266,11 +1339,11 @@ static size_t xform_lds_bytes(int dtype, const PassDesc& h) {
 }
 
 template <typename T, int KIND>
-static void launch_xform(const PassDesc* d_desc, int ept, dim3 grid, size_t lds, hipStream_t s) {
+static void launch_xform(const PassDesc* d_desc, int ept, dim3 grid, size_t lds, int64_t nt, int ib, hipStream_t s) {
   switch (ept) {
-    case 4: xform_pass_kernel<T, KIND, 4><<<grid, NTHREADS, lds, s>>>(d_desc); break;
-    case 8: xform_pass_kernel<T, KIND, 8><<<grid, NTHREADS, lds, s>>>(d_desc); break;
-    default: xform_pass_kernel<T, KIND, 16><<<grid, NTHREADS, lds, s>>>(d_desc); break;
+    case 4: xform_pass_kernel<T, KIND, 4><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
+    case 8: xform_pass_kernel<T, KIND, 8><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
+    default: xform_pass_kernel<T, KIND, 16><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
   }
 }
 
@@ -1287,12 +1360,14 @@ static void set_lds_attr() {
 int sod_pass_lds_bytes(int dtype, const PassDesc& h) { return (int)xform_lds_bytes(dtype, h); }
 
 int sod_launch_pass(int dtype, const PassDesc* d_desc, const PassDesc& h, hipStream_t s) {
-  const dim3 grid((unsigned)h.ntiles), block(NTHREADS);
+  const int ib = h.inherit_block && h.prev_nscalar > 0 ? 1 : 0;
+  const int64_t nt = h.ntiles;
+  const dim3 grid((unsigned)(nt + ib)), block(NTHREADS);
   if (h.nx == 0) {
     if (dtype)
-      elem_pass_kernel<double, SOD_ELEM_EPT><<<grid, block, 0, s>>>(d_desc);
+      elem_pass_kernel<double, SOD_ELEM_EPT><<<grid, block, 0, s>>>(d_desc, nt, ib);
     else
-      elem_pass_kernel<float, SOD_ELEM_EPT><<<grid, block, 0, s>>>(d_desc);
+      elem_pass_kernel<float, SOD_ELEM_EPT><<<grid, block, 0, s>>>(d_desc, nt, ib);
   } else {
     const size_t lds = xform_lds_bytes(dtype, h);
     static bool attr_set = false;
@@ -1308,21 +1383,21 @@ int sod_launch_pass(int dtype, const PassDesc* d_desc, const PassDesc& h, hipStr
     const bool mat = h.xf[0].type == XF_MAT;
     if (dtype) {
       if (mat)
-        launch_xform<double, XF_MAT>(d_desc, h.ept, grid, lds, s);
+        launch_xform<double, XF_MAT>(d_desc, h.ept, grid, lds, nt, ib, s);
       else
-        launch_xform<double, XF_FFT>(d_desc, h.ept, grid, lds, s);
+        launch_xform<double, XF_FFT>(d_desc, h.ept, grid, lds, nt, ib, s);
     } else {
       if (mat)
-        launch_xform<float, XF_MAT>(d_desc, h.ept, grid, lds, s);
+        launch_xform<float, XF_MAT>(d_desc, h.ept, grid, lds, nt, ib, s);
       else
-        launch_xform<float, XF_FFT>(d_desc, h.ept, grid, lds, s);
+        launch_xform<float, XF_FFT>(d_desc, h.ept, grid, lds, nt, ib, s);
     }
   }
   return (int)hipGetLastError();
 }
 
 template <typename T, int KIND>
-static void launch_xform_batch(const PassDesc* const* ds, const int64_t* starts, int ninst, int ept, dim3 grid,
+static void launch_xform_batch(const PassDesc* ds, const int64_t* starts, int ninst, int ept, dim3 grid,
                                size_t lds, hipStream_t s) {
   switch (ept) {
     case 4: xform_batch_kernel<T, KIND, 4><<<grid, NTHREADS, lds, s>>>(ds, starts, ninst); break;
@@ -1341,7 +1416,7 @@ static void set_lds_attr_batch() {
                             (int)SOD_MAX_DYN_LDS);
 }
 
-int sod_launch_batch(int dtype, int nx, int xf_type, const PassDesc* const* d_descs, const int64_t* d_starts,
+int sod_launch_batch(int dtype, int nx, int xf_type, const PassDesc* d_descs, const int64_t* d_starts,
                      int ninst, int64_t total_tiles, int ept, size_t lds, hipStream_t s) {
   const dim3 grid((unsigned)total_tiles);
   if (nx == 0) {

@@ -141,25 +141,25 @@ __global__ void bias_act_kernel(const float* Y, const float* __restrict__ b, int
   }
 }
 
-// out = base + Z (the prediction)
-__global__ void add_base_kernel(const float* __restrict__ Z, const float* __restrict__ base, int64_t rows,
+// out = base + s Z (the prediction)
+__global__ void add_base_kernel(const float* __restrict__ Z, const float* __restrict__ base, float s, int64_t rows,
                                 int64_t cols, float* __restrict__ out) {
   const int64_t n = rows * cols;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    out[i] = Z[i] + base[i % cols];
+    out[i] = base[i % cols] + s * Z[i];
 }
 
-// Mean squared error of base + Z against T: dZ = 2 (base + Z - T) / n and
-// per-block partial sums of the squared error (FP64)
-__global__ __launch_bounds__(EW) void mse_kernel(const float* __restrict__ Z, const float* __restrict__ base,
+// Mean squared error of base + s Z against T: dZ = 2 s (base + s Z - T) / n
+// and per-block partial sums of the squared error (FP64)
+__global__ __launch_bounds__(EW) void mse_kernel(const float* __restrict__ Z, const float* __restrict__ base, float sc,
                                                  const float* __restrict__ T, int64_t rows, int64_t cols,
                                                  float* __restrict__ dZ, double* __restrict__ partial) {
   __shared__ double red[EW];
   const int64_t n = rows * cols;
-  const float g = 2.0f / (float)n;
+  const float g = 2.0f * sc / (float)n;
   double s = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)EW + threadIdx.x; i < n; i += (int64_t)gridDim.x * EW) {
-    const float d = Z[i] + base[i % cols] - T[i];
+    const float d = base[i % cols] + sc * Z[i] - T[i];
     dZ[i] = g * d;
     s += (double)d * (double)d;
   }
@@ -223,6 +223,7 @@ struct so_mlp {
   std::vector<float*> W, b, gW, gb, mW, vW, mb, vb;
   std::vector<uint16_t*> Wh;
   float* base = nullptr;
+  float oscale = 1.0f;  // prediction = base + oscale * net(x)
   int64_t cap = 0;
   std::vector<float*> Z;  // Z[l] for l = 1..L: [cap][w[l]]
   float* G1 = nullptr;
@@ -332,7 +333,7 @@ double backward(so_mlp* m, const float* x, const float* t, int64_t batch) {
   const int L = m->L;
   const int64_t nout = m->w[L];
   const unsigned lb = std::min<unsigned>(1024, grid_for(batch * nout));
-  mse_kernel<<<lb, EW, 0, m->ctx->stream>>>(m->Z[L], m->base, t, batch, nout, m->G1, m->lpart);
+  mse_kernel<<<lb, EW, 0, m->ctx->stream>>>(m->Z[L], m->base, m->oscale, t, batch, nout, m->G1, m->lpart);
   float* G = m->G1;  // dLoss/dZ[l + 1], then dLoss/dY[l]
   float* H = m->G2;
   for (int l = L - 1; l >= 0; --l) {
@@ -440,6 +441,14 @@ extern "C" int so_mlp_set_base(so_mlp* m, const float* base) {
   API_END
 }
 
+extern "C" int so_mlp_set_output_scale(so_mlp* m, float scale) {
+  API_BEGIN
+  NEED(m);
+  if (!(scale > 0.0f) || !std::isfinite(scale)) fail(SO_ERR_INVALID, "output scale must be positive and finite");
+  m->oscale = scale;
+  API_END
+}
+
 extern "C" int so_mlp_params(so_mlp* m, int layer, float* w, float* b) {
   API_BEGIN
   NEED(m);
@@ -473,7 +482,7 @@ extern "C" int so_mlp_forward(so_mlp* m, const float* x, int64_t batch, float* y
   if (batch < 1) fail(SO_ERR_INVALID, "batch must be positive");
   forward(m, x, batch);
   const int64_t nout = m->w[m->L];
-  add_base_kernel<<<grid_for(batch * nout), EW, 0, m->ctx->stream>>>(m->Z[m->L], m->base, batch, nout, y);
+  add_base_kernel<<<grid_for(batch * nout), EW, 0, m->ctx->stream>>>(m->Z[m->L], m->base, m->oscale, batch, nout, y);
   HIPCHK(hipGetLastError());
   API_END
 }

@@ -937,6 +937,7 @@ CompiledPass* Propagation::compile(const PassB& b, bool immediate) {
         sod::ElemOp& o = h.st[s].op[i];
         o.lds = -1;
         o.pf = -1;
+        o.lf = -1;
         o.kind = e.kind;
         o.axis = e.axis;
         o.table = e.table;
@@ -981,10 +982,29 @@ CompiledPass* Propagation::compile(const PassB& b, bool immediate) {
         h.pf_src[h.n_pf] = src;
         return h.n_pf++;
       };
+      // 1-D tables off the lane axis give one factor per lane; EK_SCALE
+      // factors are final before the launch: both are read once at kernel start
+      auto add_lf = [&](const void* src, int ax) -> int {
+        for (int i = 0; i < h.n_lf; ++i)
+          if (h.lf_src[i] == src && h.lf_axis[i] == ax) return i;
+        if (h.n_lf >= sod::MAX_LF) return -1;
+        h.lf_src[h.n_lf] = src;
+        h.lf_axis[h.n_lf] = ax;
+        return h.n_lf++;
+      };
+      auto add_hs = [&](int slot) -> int {
+        for (int i = 0; i < h.n_hs; ++i)
+          if (h.hs_slot[i] == slot) return i;
+        if (h.n_hs >= sod::MAX_HS) return -1;
+        h.hs_slot[h.n_hs] = slot;
+        return h.n_hs++;
+      };
       for (int st = 0; st < 3; ++st)
         for (int i = 0; i < h.st[st].n; ++i) {
           sod::ElemOp& o = h.st[st].op[i];
           if (o.kind == sod::EK_MUL1D && o.axis == axis) o.lds = add_arr(o.table, n * (int64_t)cs);
+          if (o.kind == sod::EK_MUL1D && o.axis != axis) o.lf = add_lf(o.table, o.axis);
+          if (o.kind == sod::EK_SCALE) o.lf = add_hs(o.slot);
           if (o.kind == sod::EK_NORM) o.lds = add_arr(o.w_axis, n * 8);
           if (o.kind == sod::EK_MULND) o.pf = add_pf(o.table);
         }
@@ -1001,6 +1021,12 @@ CompiledPass* Propagation::compile(const PassB& b, bool immediate) {
     }
   }
   h.fin_mode = h.has_norm ? (immediate ? sod::FIN_IMMEDIATE : sod::FIN_DEFERRED) : sod::FIN_NONE;
+  // the inherited program gets a workgroup of its own unless this pass runs a
+  // program of its own inside the launch (which must follow the inherited one)
+  h.inherit_block = (h.prev_nscalar > 0 && !scalar_only && h.fin_mode != sod::FIN_IMMEDIATE &&
+                     !(h.has_norm == 0 && h.nscalar > 0))
+                        ? 1
+                        : 0;
   cp->d = (sod::PassDesc*)ctx->alloc(sizeof(sod::PassDesc));
   HIPCHK(hipMemcpy(cp->d, &h, sizeof(sod::PassDesc), hipMemcpyHostToDevice));
   CompiledPass* raw = cp.get();
@@ -2346,7 +2372,7 @@ extern "C" int so_propagation_destroy(so_propagation* pr) {
 // its own stack, and the k-th pass of all members runs as one kernel launch
 // (workgroups find their member's descriptor from a prefix sum of tiles).
 struct BatchLaunch {
-  sod::PassDesc** d_descs = nullptr;
+  sod::PassDesc* d_descs = nullptr;  // copies of the members' descriptors, contiguous
   int64_t* d_starts = nullptr;
   int64_t total = 0;
   int ept = 4;
@@ -2465,10 +2491,10 @@ static BatchLaunch* batch_launch_for(so_batch* b, const std::vector<CompiledPass
   auto it = b->cache.find(key);
   if (it != b->cache.end()) return &it->second;
   BatchLaunch bl;
-  std::vector<sod::PassDesc*> ds;
+  std::vector<sod::PassDesc> ds;
   std::vector<int64_t> starts(1, 0);
   for (auto* cp : passes) {
-    ds.push_back(cp->d);
+    ds.push_back(cp->h);
     starts.push_back(starts.back() + cp->h.ntiles);
     bl.ept = std::max(bl.ept, cp->h.ept);
     bl.lds = std::max(bl.lds, (size_t)cp->h.lds_bytes);
@@ -2476,9 +2502,9 @@ static BatchLaunch* batch_launch_for(so_batch* b, const std::vector<CompiledPass
   bl.total = starts.back();
   bl.nx = passes[0]->h.nx;
   bl.xf_type = bl.nx ? passes[0]->h.xf[0].type : 0;
-  bl.d_descs = (sod::PassDesc**)b->ctx->alloc(sizeof(void*) * ds.size());
+  bl.d_descs = (sod::PassDesc*)b->ctx->alloc(sizeof(sod::PassDesc) * ds.size());
   bl.d_starts = (int64_t*)b->ctx->alloc(sizeof(int64_t) * starts.size());
-  HIPCHK(hipMemcpy(bl.d_descs, ds.data(), sizeof(void*) * ds.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(bl.d_descs, ds.data(), sizeof(sod::PassDesc) * ds.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(bl.d_starts, starts.data(), sizeof(int64_t) * starts.size(), hipMemcpyHostToDevice));
   return &b->cache.emplace(key, bl).first->second;
 }

@@ -42,7 +42,8 @@ def device_stream():
     if _stream is None:
         import torch
 
-        _stream = torch.cuda.ExternalStream(so.stream_handle())
+        torch.cuda.init()
+        _stream = torch.cuda.ExternalStream(so.stream_handle(), device=torch.device("cuda", torch.cuda.current_device()))
     return _stream
 
 
@@ -63,7 +64,7 @@ def to_device(a):
 
 
 class MLP:
-    """Fully connected tanh network with linear output: y = base + net(x)."""
+    """Fully connected tanh network with linear output: y = base + scale net(x)."""
 
     def __init__(self, widths, seed=5):
         self.widths = [int(w) for w in widths]
@@ -80,6 +81,9 @@ class MLP:
         b = np.ascontiguousarray(base, dtype=np.float32)
         assert b.size == self.widths[-1]
         call("so_mlp_set_base", self._h, _fptr(b))
+
+    def set_output_scale(self, scale):
+        call("so_mlp_set_output_scale", self._h, C.c_float(scale))
 
     def params(self, layer):
         w = np.zeros((self.widths[layer + 1], self.widths[layer]), np.float32)
@@ -205,9 +209,11 @@ class LearnedWarmStart:
         return out
 
     def fit(self, params, targets, epochs=200, minibatch=256, lr=1e-3, lr_final=1e-5, seed=7, log=None):
-        """Adam on mean squared error; the output base is the mean target, the network learns the rest."""
+        """Adam on mean squared error; the output base is the mean target and the network learns the rest
+        in units of its RMS (output scale), so its outputs are O(1)."""
         base = targets.mean(axis=0)
         self.net.set_base(base)
+        self.net.set_output_scale(float(np.sqrt(np.mean((targets - base) ** 2))) or 1.0)
         x = to_device(normalise_params(params))
         t = to_device(targets)
         n = len(params)

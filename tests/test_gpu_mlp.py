@@ -65,11 +65,12 @@ def test_forward_matches_oracle(widths):
     assert rel_l2(y.cpu().numpy(), ref) < 1e-5
 
 
-@pytest.mark.parametrize("widths", WIDTHS)
-def test_gradients_match_oracle(widths):
+@pytest.mark.parametrize("widths,scale", [(w, 1.0) for w in WIDTHS] + [(WIDTHS[1], 0.01)])
+def test_gradients_match_oracle(widths, scale):
     net, params, base, x, t = _setup(widths, 37)
+    net.set_output_scale(scale)
     loss = net.backward(WS.to_device(x), WS.to_device(t))
-    ref_loss, ref = MO.loss_and_grads(params, x.astype(np.float64), t.astype(np.float64), base)
+    ref_loss, ref = MO.loss_and_grads(params, x.astype(np.float64), t.astype(np.float64), base, scale=scale)
     assert abs(loss - ref_loss) < 1e-5 * ref_loss
     for l in range(len(widths) - 1):
         gw, gb = net.gradient(l)

@@ -19,7 +19,8 @@ def test_gradients_match_central_differences():
     x = rng.uniform(-1, 1, (5, 3))
     t = rng.normal(0, 1, (5, 4))
     base = rng.normal(0, 0.1, 4)
-    loss, grads = M.loss_and_grads(params, x, t, base, bf=False)
+    sc = 0.3
+    loss, grads = M.loss_and_grads(params, x, t, base, bf=False, scale=sc)
     h = 1e-6
     for l, (w, b) in enumerate(params):
         for arr, g in ((w, grads[l][0]), (b, grads[l][1])):
@@ -29,15 +30,15 @@ def test_gradients_match_central_differences():
                 i = it.multi_index
                 old = arr[i]
                 arr[i] = old + h
-                lp, _ = M.loss_and_grads(params, x, t, base, bf=False)
+                lp, _ = M.loss_and_grads(params, x, t, base, bf=False, scale=sc)
                 arr[i] = old - h
-                lm, _ = M.loss_and_grads(params, x, t, base, bf=False)
+                lm, _ = M.loss_and_grads(params, x, t, base, bf=False, scale=sc)
                 arr[i] = old
                 num[i] = (lp - lm) / (2 * h)
             assert np.max(np.abs(num - g)) < 1e-7 * max(1.0, np.max(np.abs(g))), (l, np.max(np.abs(num - g)))
-    # the loss is the mean squared error of base + net(x)
+    # the loss is the mean squared error of base + scale net(x)
     z = M.forward(params, x, bf=False)[-1]
-    assert abs(loss - np.mean((z + base - t) ** 2)) < 1e-15
+    assert abs(loss - np.mean((base + sc * z - t) ** 2)) < 1e-15
 
 
 def test_bf16_rounding():
