@@ -42,9 +42,10 @@ def parse():
     ap.add_argument("--c4-f32-steps", type=int, default=2000, help="steps of the FP32 batched throughput run")
     ap.add_argument("--c4-solo", action="store_true", help="also relax the instances one by one")
     ap.add_argument("--c5-train", type=int, default=1024, help="training potentials of the warm start (0: skip)")
-    ap.add_argument("--c5-train-steps", type=int, default=20480, help="imaginary-time steps per training solution")
+    ap.add_argument("--c5-cold-steps", type=int, default=28672, help="steps of the cold-started training solutions")
+    ap.add_argument("--c5-warm-steps", type=int, default=10240, help="steps of the warm-started training solutions")
     ap.add_argument("--c5-eval", type=int, default=64, help="held-out potentials of the warm-start evaluation")
-    ap.add_argument("--c5-epochs", type=int, default=300)
+    ap.add_argument("--c5-updates", type=int, default=4000, help="optimiser steps per fit")
     return ap.parse_args()
 
 
@@ -169,13 +170,10 @@ def learned_warm_start(so, args):
     ptrain = lo + (hi - lo) * rng.uniform(size=(args.c5_train, 3))
     peval = lo + (hi - lo) * np.random.default_rng(6000).uniform(size=(args.c5_eval, 3))
     t0 = time.perf_counter()
-    targets = ws.generate(ptrain, args.c5_train_steps, batch=256, log=log)
+    targets = ws.train(ptrain, args.c5_cold_steps, args.c5_warm_steps, first=256, updates=args.c5_updates, log=log)
     so.synchronize()
-    t_gen = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    losses = ws.fit(ptrain, targets, epochs=args.c5_epochs, minibatch=256, lr=1e-3, lr_final=1e-5, log=log)
-    so.synchronize()
-    t_fit = time.perf_counter() - t0
+    t_train = time.perf_counter() - t0
+    losses = ws.losses
     del targets
     log("reference run")
     ref, _ = ws.relax(peval, 36864)
@@ -194,9 +192,10 @@ def learned_warm_start(so, args):
     return {
         "grid": [1024, 32], "dt": 0.05, "storage": "complex f64 propagation, FP32 training solutions",
         "network": {"widths": ws.widths, "weights": "bf16 (fp32 Adam masters)", "accumulate": "fp32"},
-        "train": {"potentials": args.c5_train, "steps_per_solution": args.c5_train_steps,
-                  "generate_s": t_gen, "epochs": args.c5_epochs, "fit_s": t_fit, "loss_first": losses[0],
-                  "loss_last": losses[-1]},
+        "train": {"potentials": args.c5_train, "curriculum": "first 256 cold, the rest warm-started from the first fit",
+                  "cold_steps": args.c5_cold_steps, "warm_steps": args.c5_warm_steps, "updates_per_fit": args.c5_updates,
+                  "generate_and_fit_s": t_train, "loss_first": losses[0], "loss_last": losses[-1],
+                  "loss_unit": "per sample, sum over outputs of (error / output scale)^2"},
         "eval_potentials": args.c5_eval, "criterion": "|E - E_ref| <= 1e-8 |E_ref|, E_ref from 36864 cold steps",
         "cold": {"time_s": t_cold, "all_met": ok_c, "steps_max": max(it["steps"] for it in cold),
                  "steps_mean": float(np.mean([it["steps"] for it in cold]))},

@@ -232,7 +232,8 @@ int so_mlp_set_output_scale(so_mlp *m, float scale);                            
 int so_mlp_params(so_mlp *m, int layer, float *w, float *b);                    /* read FP32 masters (host) */
 int so_mlp_set_params(so_mlp *m, int layer, const float *w, const float *b);
 int so_mlp_forward(so_mlp *m, const float *x, int64_t batch, float *y);         /* y = base + scale net(x) */
-/* gradients of mean((base + scale net(x) - target)^2) into the network's gradient buffers; *loss = that mean */
+/* gradients of the loss (1/batch) sum over samples and outputs of ((base + scale net(x) - target) / scale)^2 into the
+ * network's gradient buffers; *loss = that loss */
 int so_mlp_backward(so_mlp *m, const float *x, const float *target, int64_t batch, double *loss);
 int so_mlp_gradient(so_mlp *m, int layer, float *gw, float *gb);                /* read gradients (host) */
 int so_mlp_adam(so_mlp *m, double lr, double beta1, double beta2, double eps);  /* one Adam step on the gradients */

@@ -42,14 +42,15 @@ def forward(params, x, bf=True):
 
 
 def loss_and_grads(params, x, target, base, bf=True, scale=1.0):
-    """mean((base + scale net(x) - target)^2) and its gradients [(gW, gb), ...] per layer."""
+    """Loss (1/batch) sum ((base + scale net(x) - target) / scale)^2 (sum over outputs, mean over samples)
+    and its gradients [(gW, gb), ...] per layer."""
     q = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
     zs = forward(params, x, bf)
     L = len(params)
-    d = _f32(np.asarray(base, dtype=np.float64) + scale * zs[-1] - np.asarray(target, dtype=np.float64), bf)
-    n = d.size
-    loss = float(np.mean(d * d))
-    g = _f32(d * (2.0 * scale / n), bf)
+    d = _f32((np.asarray(base, dtype=np.float64) + scale * zs[-1] - np.asarray(target, dtype=np.float64)) / scale, bf)
+    rows = d.shape[0]
+    loss = float(np.sum(d * d) / rows)
+    g = _f32(d * (2.0 / rows), bf)
     grads = [None] * L
     for l in range(L - 1, -1, -1):
         if l < L - 1:

@@ -149,19 +149,22 @@ __global__ void add_base_kernel(const float* __restrict__ Z, const float* __rest
     out[i] = base[i % cols] + s * Z[i];
 }
 
-// Mean squared error of base + s Z against T: dZ = 2 s (base + s Z - T) / n
-// and per-block partial sums of the squared error (FP64)
+// Squared error of the prediction base + s Z against T in units of the output
+// scale s, summed over the outputs of a sample: with e = (base + s Z - T) / s,
+// dZ = 2 e / rows (the loss is averaged over the rows of the batch) and
+// per-block partial sums of e^2 (FP64)
 __global__ __launch_bounds__(EW) void mse_kernel(const float* __restrict__ Z, const float* __restrict__ base, float sc,
                                                  const float* __restrict__ T, int64_t rows, int64_t cols,
                                                  float* __restrict__ dZ, double* __restrict__ partial) {
   __shared__ double red[EW];
   const int64_t n = rows * cols;
-  const float g = 2.0f * sc / (float)n;
+  const float g = 2.0f / (float)rows;
+  const float inv = 1.0f / sc;
   double s = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)EW + threadIdx.x; i < n; i += (int64_t)gridDim.x * EW) {
-    const float d = base[i % cols] + sc * Z[i] - T[i];
-    dZ[i] = g * d;
-    s += (double)d * (double)d;
+    const float e = (base[i % cols] + sc * Z[i] - T[i]) * inv;
+    dZ[i] = g * e;
+    s += (double)e * (double)e;
   }
   red[threadIdx.x] = s;
   __syncthreads();
@@ -355,7 +358,7 @@ double backward(so_mlp* m, const float* x, const float* t, int64_t batch) {
   m->ctx->sync();
   double s = 0.0;
   for (double v : lp) s += v;
-  return s / (double)(batch * nout);
+  return s / (double)batch;
 }
 
 void adam(so_mlp* m, double lr, double b1, double b2, double eps) {

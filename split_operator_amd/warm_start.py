@@ -138,9 +138,12 @@ class MLP:
         return Cm
 
     def __del__(self):
-        if getattr(self, "_h", None) and so._lib._lib is not None:
-            so._lib._lib.so_mlp_destroy(self._h)
-            self._h = None
+        try:
+            if getattr(self, "_h", None) and so._lib._lib is not None:
+                so._lib._lib.so_mlp_destroy(self._h)
+        except (AttributeError, TypeError):  # interpreter shutdown
+            pass
+        self._h = None
 
 
 def normalise_params(p):
@@ -163,6 +166,7 @@ class LearnedWarmStart:
     n_theta Gauss-Legendre nodes, l = 0..n_theta-1), imaginary time step dt."""
 
     def __init__(self, n_r=1024, n_theta=32, dt=0.05, hidden=(256, 256, 256), seed=5):
+        device_stream()  # torch's device state is set up before the solver's work
         self.n_r, self.n_theta, self.dt = n_r, n_theta, dt
         self.gl = so.gauss_legendre(n_theta)
         theta = np.arccos(np.asarray(self.gl[0]))[::-1].copy()
@@ -171,6 +175,7 @@ class LearnedWarmStart:
         self.t, self.tinv = so.legendre_matrices(polar)
         self.radial = so.Grid.new_linear_continuos("radial", 0.5, 50.0, n_r, 0)
         self.widths = [3, *hidden, 2 * n_r * n_theta]
+        self.seed = seed
         self.net = MLP(self.widths, seed=seed)
         self.losses = []
 
@@ -187,12 +192,14 @@ class LearnedWarmStart:
                                    leak_frames=leak_frames, psi0=psi0)
         return p
 
-    def generate(self, params, steps, batch=256, dtype="f32", log=None):
-        """Relax each potential from the cold start for `steps` imaginary-time steps (batched); returns
-        the solutions psi(r, l) as rows of interleaved re/im FP32."""
+    def generate(self, params, steps, batch=256, dtype="f32", init=None, log=None):
+        """Relax each potential for `steps` imaginary-time steps (batched) from the cold start, or from
+        init[i] (psi(r, theta)) when given; returns the solutions psi(r, l) as rows of interleaved re/im
+        FP32."""
         out = np.zeros((len(params), 2 * self.n_r * self.n_theta), np.float32)
         for s in range(0, len(params), batch):
-            props = [self._instance(pp, steps, dtype) for pp in params[s:s + batch]]
+            props = [self._instance(pp, steps, dtype, None if init is None else init[s + i])
+                     for i, pp in enumerate(params[s:s + batch])]
             b = so.Batch()
             for p in props:
                 b.add(p)
@@ -208,9 +215,28 @@ class LearnedWarmStart:
                 log(f"generated {min(s + batch, len(params))}/{len(params)}")
         return out
 
+    def train(self, params, steps_cold, steps_warm, first=256, updates=4000, minibatch=256, log=None):
+        """Curriculum on the solver's own solutions: the first `first` potentials are relaxed from the
+        cold start for `steps_cold` steps and fitted; the rest are relaxed for `steps_warm` steps from
+        that network's predictions, and the network is fitted again on all of them. Returns the targets."""
+        first = min(first, len(params))
+        t1 = self.generate(params[:first], steps_cold, log=log)
+        if first == len(params):
+            self.fit(params, t1, epochs=max(1, updates * minibatch // first), minibatch=minibatch, log=log)
+            return t1
+        self.fit(params[:first], t1, epochs=max(1, updates * minibatch // first), minibatch=minibatch, log=log)
+        rest = params[first:]
+        t2 = self.generate(rest, steps_warm, init=self.predict(rest), log=log)
+        targets = np.concatenate([t1, t2])
+        del t1, t2
+        self.net = MLP(self.widths, seed=self.seed)
+        self.losses = []
+        self.fit(params, targets, epochs=max(1, updates * minibatch // len(params)), minibatch=minibatch, log=log)
+        return targets
+
     def fit(self, params, targets, epochs=200, minibatch=256, lr=1e-3, lr_final=1e-5, seed=7, log=None):
-        """Adam on mean squared error; the output base is the mean target and the network learns the rest
-        in units of its RMS (output scale), so its outputs are O(1)."""
+        """Adam on the squared error; the output base is the mean target and the network learns the rest
+        in units of its RMS (output scale), so its outputs and the loss per sample are O(1)."""
         base = targets.mean(axis=0)
         self.net.set_base(base)
         self.net.set_output_scale(float(np.sqrt(np.mean((targets - base) ** 2))) or 1.0)

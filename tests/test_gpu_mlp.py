@@ -71,7 +71,7 @@ def test_gradients_match_oracle(widths, scale):
     net.set_output_scale(scale)
     loss = net.backward(WS.to_device(x), WS.to_device(t))
     ref_loss, ref = MO.loss_and_grads(params, x.astype(np.float64), t.astype(np.float64), base, scale=scale)
-    assert abs(loss - ref_loss) < 1e-5 * ref_loss
+    assert abs(loss - ref_loss) < 2e-5 * ref_loss
     for l in range(len(widths) - 1):
         gw, gb = net.gradient(l)
         assert rel_l2(gw, ref[l][0]) < 1e-3, l

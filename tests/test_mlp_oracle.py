@@ -36,9 +36,9 @@ def test_gradients_match_central_differences():
                 arr[i] = old
                 num[i] = (lp - lm) / (2 * h)
             assert np.max(np.abs(num - g)) < 1e-7 * max(1.0, np.max(np.abs(g))), (l, np.max(np.abs(num - g)))
-    # the loss is the mean squared error of base + scale net(x)
+    # the loss is the per-sample squared error in units of the output scale
     z = M.forward(params, x, bf=False)[-1]
-    assert abs(loss - np.mean((base + sc * z - t) ** 2)) < 1e-15
+    assert abs(loss - np.sum(((base + sc * z - t) / sc) ** 2) / len(x)) < 1e-12
 
 
 def test_bf16_rounding():
