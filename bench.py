@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--c5-warm-steps", type=int, default=10240, help="steps of the warm-started training solutions")
     ap.add_argument("--c5-eval", type=int, default=64, help="held-out potentials of the warm-start evaluation")
     ap.add_argument("--c5-updates", type=int, default=4000, help="optimiser steps per fit")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy a MIB-MiB buffer 3 times (a plain copy kernel of known traffic for "
+                         "scaling hardware byte counters), then run the benchmark")
     return ap.parse_args()
 
 
@@ -217,6 +220,14 @@ def main():
 
     import split_operator_amd as so
     from split_operator_amd import workloads as W
+
+    if args.copy_calibration > 0:
+        src = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     seed = 1 * 1000 + 0
     p, (radial, polar), (c6, c12, a) = W.radial_angular(
