@@ -33,7 +33,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--cpu-steps", type=int, default=20, help="steps of the CPU oracle baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=1000, help="steps of the CPU oracle baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-iters", type=int, default=200)
     ap.add_argument("--c3-steps", type=int, default=100, help="steps of the 256^3 Cartesian workload (0: skip)")
@@ -147,11 +147,15 @@ def ground_states(so, W, count, max_steps, f32_steps, solo):
     g64 = W.GroundStateBatch(so, count=count, seed0=4000, dt=dt, max_steps=f32_steps, tol=0.0, dtype="f64")
     g64.advance(f32_steps)
     g64.check()
-    errs = [abs(a - it["energy"]) / abs(it["energy"]) for a, it in zip(e32, g64.inst)]
+    import numpy as np
+
+    diffs = [abs(a - it["energy"]) for a, it in zip(e32, g64.inst)]
+    rels = [d / abs(it["energy"]) for d, it in zip(diffs, g64.inst)]
     del g64
     rec["f32"] = {"steps": f32_steps - 16, "pt_steps_per_s": rec["points"] * (f32_steps - 16) / wall,
                   "ms_per_step": wall / (f32_steps - 16) * 1e3,
-                  "max_rel_energy_diff_vs_f64_same_steps": max(errs)}
+                  "max_abs_energy_diff_vs_f64_same_steps": max(diffs),
+                  "median_rel_energy_diff_vs_f64_same_steps": float(np.median(rels))}
     return rec
 
 
@@ -245,6 +249,16 @@ def main():
     value = pts * args.steps / wall
     stats = p.stats()
 
+    # FP32 against FP64 storage after exactly 1000 steps of the same stack (the accuracy target)
+    l2_1000 = []
+    for dt_ in ("f32", "f64"):
+        r, _, _ = W.radial_angular(so, n_r=N_R, n_theta=N_THETA, dt=DT, steps=1000, seed=seed, dtype=dt_,
+                                   gl=so.gauss_legendre(N_THETA))
+        r.steps(1000)
+        l2_1000.append(r.wave_function().array)
+        del r
+    l2_1000 = float(np.linalg.norm(l2_1000[0] - l2_1000[1]) / np.linalg.norm(l2_1000[1]))
+
     # accuracy of the FP32 run against the same stack stored in FP64 on the GPU
     q, _, _ = W.radial_angular(so, n_r=N_R, n_theta=N_THETA, dt=DT, steps=args.steps, seed=seed, dtype="f64",
                                gl=so.gauss_legendre(N_THETA))
@@ -303,6 +317,7 @@ def main():
         "cpu_baseline": cpu,
         "launches_per_step": stats["launches_per_step"],
         "rel_l2_vs_f64_gpu": l2,
+        "rel_l2_vs_f64_gpu_1000_steps": l2_1000,
         "passes": passes,
         "extra_workloads": extra,
     }
