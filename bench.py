@@ -249,15 +249,28 @@ def main():
     value = pts * args.steps / wall
     stats = p.stats()
 
-    # FP32 against FP64 storage after exactly 1000 steps of the same stack (the accuracy target)
-    l2_1000 = []
-    for dt_ in ("f32", "f64"):
+    # FP32 and FP32-storage/FP64-arithmetic against FP64 storage after exactly 1000 steps of the same stack
+    fin = {}
+    for dt_ in ("f32", "f32x", "f64"):
         r, _, _ = W.radial_angular(so, n_r=N_R, n_theta=N_THETA, dt=DT, steps=1000, seed=seed, dtype=dt_,
                                    gl=so.gauss_legendre(N_THETA))
         r.steps(1000)
-        l2_1000.append(r.wave_function().array)
+        fin[dt_] = r.wave_function().array
         del r
-    l2_1000 = float(np.linalg.norm(l2_1000[0] - l2_1000[1]) / np.linalg.norm(l2_1000[1]))
+    l2_1000 = float(np.linalg.norm(fin["f32"] - fin["f64"]) / np.linalg.norm(fin["f64"]))
+    l2x_1000 = float(np.linalg.norm(fin["f32x"] - fin["f64"]) / np.linalg.norm(fin["f64"]))
+    del fin
+
+    # the same workload with FP32 storage and FP64 pass arithmetic: throughput
+    px, _, _ = W.radial_angular(so, n_r=N_R, n_theta=N_THETA, dt=DT, steps=args.steps, seed=seed, dtype="f32x",
+                                gl=so.gauss_legendre(N_THETA))
+    px.steps(args.warmup)
+    so.synchronize()
+    t0 = time.perf_counter()
+    px.steps(args.steps)
+    so.synchronize()
+    wall_x = time.perf_counter() - t0
+    del px
 
     # accuracy of the FP32 run against the same stack stored in FP64 on the GPU
     q, _, _ = W.radial_angular(so, n_r=N_R, n_theta=N_THETA, dt=DT, steps=args.steps, seed=seed, dtype="f64",
@@ -318,6 +331,9 @@ def main():
         "launches_per_step": stats["launches_per_step"],
         "rel_l2_vs_f64_gpu": l2,
         "rel_l2_vs_f64_gpu_1000_steps": l2_1000,
+        "f32_storage_f64_arithmetic": {"value": N_R * N_THETA * args.steps / wall_x,
+                                       "ms_per_step": wall_x / args.steps * 1e3,
+                                       "rel_l2_vs_f64_gpu_1000_steps": l2x_1000},
         "passes": passes,
         "extra_workloads": extra,
     }

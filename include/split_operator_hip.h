@@ -40,7 +40,9 @@ enum {
     SO_ERR_STATE = -8        /* e.g. propagation without a wave function */
 };
 
-enum { SO_COMPLEX_F32 = 0, SO_COMPLEX_F64 = 1 };                /* device storage of psi */
+/* device storage of psi: complex FP32, complex FP64, or complex FP32 storage with FP64 arithmetic in the passes
+ * (tables, twiddles, Legendre matrices and LDS tiles in FP64; HBM traffic of FP32) */
+enum { SO_COMPLEX_F32 = 0, SO_COMPLEX_F64 = 1, SO_COMPLEX_F32_F64_ARITH = 2 };
 enum { SO_APPLY_NONE = 0, SO_APPLY_FIRST_HALF = 1, SO_APPLY_SECOND_HALF = 2 }; /* control.rs:19-26 */
 enum { SO_ORDER_NORMAL = 0, SO_ORDER_INVERSE_FIRST = 1 };        /* transformation.rs:13-17 */
 

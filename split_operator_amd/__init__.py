@@ -28,15 +28,18 @@ ORDER_NORMAL = 0
 ORDER_INVERSE_FIRST = 1
 COMPLEX_F32 = 0
 COMPLEX_F64 = 1
+COMPLEX_F32_F64_ARITH = 2
 
 _DTYPES = {"f32": COMPLEX_F32, "f64": COMPLEX_F64, "complex64": COMPLEX_F32, "complex128": COMPLEX_F64,
-           COMPLEX_F32: COMPLEX_F32, COMPLEX_F64: COMPLEX_F64}
+           "f32x": COMPLEX_F32_F64_ARITH, COMPLEX_F32: COMPLEX_F32, COMPLEX_F64: COMPLEX_F64,
+           COMPLEX_F32_F64_ARITH: COMPLEX_F32_F64_ARITH}
 _default_dtype = COMPLEX_F64
 _ctx = None
 
 
 def set_default_dtype(dtype):
-    """Storage of new wave functions: "f64" (default, the crate's Complex64) or "f32"."""
+    """Storage of new wave functions: "f64" (default, the crate's Complex64), "f32", or "f32x" (complex FP32
+    storage with FP64 arithmetic inside the passes)."""
     global _default_dtype
     _default_dtype = _DTYPES[dtype]
 

@@ -154,7 +154,8 @@ enum FinMode : int32_t { FIN_NONE = 0, FIN_IMMEDIATE = 1, FIN_DEFERRED = 2 };
 }  // namespace sod
 
 // Launch entry points (kernels are in so_kernels.hip). `dtype` 0 = complex
-// FP32, 1 = complex FP64. Return hipError_t as int.
+// FP32, 1 = complex FP64; the pass and batch launchers also take 2 = complex
+// FP32 storage with FP64 arithmetic. Return hipError_t as int.
 extern "C++" {
 int sod_launch_pass(int dtype, const sod::PassDesc* d_desc, const sod::PassDesc& h_desc, hipStream_t s);
 int sod_launch_scalar(const sod::PassDesc* d_desc, hipStream_t s);

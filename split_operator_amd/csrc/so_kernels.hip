@@ -86,6 +86,15 @@ struct CT<double> {
   using T2 = double2;
 };
 
+// complex value in another precision (storage <-> arithmetic)
+template <typename D, typename V>
+__device__ __forceinline__ typename CT<D>::T2 cvt2(V v) {
+  typename CT<D>::T2 r;
+  r.x = (D)v.x;
+  r.y = (D)v.y;
+  return r;
+}
+
 template <typename T2>
 __device__ __forceinline__ T2 cmul(T2 a, T2 b) {
   T2 r;
@@ -858,13 +867,15 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t total) {
 // ---------------------------------------------------------------- the pass kernels
 // Element-wise pass (no transform): a flat, coalesced sweep; each workgroup
 // owns NTHREADS * EPT consecutive elements.
-template <typename T, int EPT>
+// S: storage precision of psi; T: arithmetic precision (tables, products, sums).
+template <typename S, typename T, int EPT>
 __device__ __forceinline__ void elem_pass_body(const PassDesc* __restrict__ d, int64_t tile, int64_t ntiles,
                                                bool inherit) {
   using T2 = typename CT<T>::T2;
+  using S2 = typename CT<S>::T2;
   const int tid = threadIdx.x;
   if (inherit) run_inherited(d, tid);
-  T2* psi = (T2*)d->psi;
+  S2* psi = (S2*)d->psi;
   const int nd = d->ndim;
   int64_t stride[MAXD];
   int64_t size = 1;
@@ -878,7 +889,7 @@ __device__ __forceinline__ void elem_pass_body(const PassDesc* __restrict__ d, i
   for (int q = 0; q < EPT; ++q) {
     const int64_t g = base + q * NTHREADS + tid;
     if (g < size) {
-      z[q] = gld(&psi[g]);
+      z[q] = cvt2<T>(gld(&psi[g]));
     } else {
       z[q].x = 0;
       z[q].y = 0;
@@ -933,14 +944,14 @@ __device__ __forceinline__ void elem_pass_body(const PassDesc* __restrict__ d, i
 #pragma unroll
   for (int q = 0; q < EPT; ++q) {
     const int64_t g = base + q * NTHREADS + tid;
-    if (g < size) gst(&psi[g], z[q]);
+    if (g < size) gst(&psi[g], cvt2<S>(z[q]));
   }
   finish_pass(d, acc, tid, tile, ntiles);
 }
 
 // With inherit_block, one workgroup beyond the ntiles tiles runs the previous
 // pass's deferred program, off the critical path of the tiles.
-template <typename T, int EPT>
+template <typename S, typename T, int EPT>
 __global__ __launch_bounds__(NTHREADS) void elem_pass_kernel(const PassDesc* __restrict__ d, int64_t ntiles,
                                                              int inherit_block) {
   const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
@@ -948,7 +959,7 @@ __global__ __launch_bounds__(NTHREADS) void elem_pass_kernel(const PassDesc* __r
     run_inherited(d, threadIdx.x);
     return;
   }
-  elem_pass_body<T, EPT>(d, b, ntiles, b == 0 && !inherit_block);
+  elem_pass_body<S, T, EPT>(d, b, ntiles, b == 0 && !inherit_block);
 }
 
 // LDS position of element k of a lane: padded in FFT passes, plain in matrix passes
@@ -987,10 +998,11 @@ __device__ __forceinline__ void map_global(int (&ql)[EPT], int (&qk)[EPT], int t
 // complex elements per thread of the tile. Every global read the pass needs
 // (psi, full-grid tables of later stages, 1-D tables and norm weights, GEMM B)
 // is issued at kernel start, so the memory latency is paid once.
-template <typename T, int KIND, int EPT>
+template <typename S, typename T, int KIND, int EPT>
 __device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, int64_t tile, int64_t ntiles,
                                                 bool inherit) {
   using T2 = typename CT<T>::T2;
+  using S2 = typename CT<S>::T2;
   constexpr int PFN = EPT <= 8 ? MAX_PF : 0;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ LaneTab lt;
@@ -1009,7 +1021,7 @@ __device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, 
   }
   setup_lanes<T>(d, lt, tile, tid);
   __syncthreads();
-  T2* psi = (T2*)d->psi;
+  S2* psi = (S2*)d->psi;
   const bool strided = inner > 1;
   int ql[EPT], qk[EPT];
   map_global<EPT>(ql, qk, tid, tot, tl, n, strided);
@@ -1018,7 +1030,8 @@ __device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, 
   for (int q = 0; q < EPT; ++q) {
     z[q].x = 0;
     z[q].y = 0;
-    if (tid + q * NTHREADS < tot && lt.valid[ql[q]]) z[q] = gld(&psi[lt.base[ql[q]] + (int64_t)qk[q] * inner]);
+    if (tid + q * NTHREADS < tot && lt.valid[ql[q]])
+      z[q] = cvt2<T>(gld(&psi[lt.base[ql[q]] + (int64_t)qk[q] * inner]));
   }
   // full-grid tables of the later stages, into registers
   T2 pf[PFN > 0 ? PFN : 1][EPT];
@@ -1107,11 +1120,12 @@ __device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, 
   stage_vec<T, EPT, PFN>(d->st[nx], z, ql, qk, tid, tot, lt, d, acc, smem, pf);
 #pragma unroll
   for (int q = 0; q < EPT; ++q)
-    if (tid + q * NTHREADS < tot && lt.valid[ql[q]]) gst(&psi[lt.base[ql[q]] + (int64_t)qk[q] * inner], z[q]);
+    if (tid + q * NTHREADS < tot && lt.valid[ql[q]])
+      gst(&psi[lt.base[ql[q]] + (int64_t)qk[q] * inner], cvt2<S>(z[q]));
   finish_pass(d, acc, tid, tile, ntiles);
 }
 
-template <typename T, int KIND, int EPT>
+template <typename S, typename T, int KIND, int EPT>
 __global__ __launch_bounds__(NTHREADS) void xform_pass_kernel(const PassDesc* __restrict__ d, int64_t ntiles,
                                                               int inherit_block) {
   const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
@@ -1119,7 +1133,7 @@ __global__ __launch_bounds__(NTHREADS) void xform_pass_kernel(const PassDesc* __
     run_inherited(d, threadIdx.x);
     return;
   }
-  xform_pass_body<T, KIND, EPT>(d, b, ntiles, b == 0 && !inherit_block);
+  xform_pass_body<S, T, KIND, EPT>(d, b, ntiles, b == 0 && !inherit_block);
 }
 
 // One launch for the same pass position of many independent wave functions
@@ -1139,22 +1153,22 @@ __device__ __forceinline__ int batch_instance(const int64_t* starts, int ninst, 
 
 // The members' descriptors are stored contiguously and indexed from the
 // kernel-argument pointer, so their fields are read with scalar loads.
-template <typename T, int KIND, int EPT>
+template <typename S, typename T, int KIND, int EPT>
 __global__ __launch_bounds__(NTHREADS) void xform_batch_kernel(const PassDesc* __restrict__ ds,
                                                                const int64_t* __restrict__ starts, int ninst) {
   const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
   const int i = batch_instance(starts, ninst, b);
   const int64_t s0 = starts[i];
-  xform_pass_body<T, KIND, EPT>(ds + i, b - s0, starts[i + 1] - s0, b == s0);
+  xform_pass_body<S, T, KIND, EPT>(ds + i, b - s0, starts[i + 1] - s0, b == s0);
 }
 
-template <typename T, int EPT>
+template <typename S, typename T, int EPT>
 __global__ __launch_bounds__(NTHREADS) void elem_batch_kernel(const PassDesc* __restrict__ ds,
                                                               const int64_t* __restrict__ starts, int ninst) {
   const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
   const int i = batch_instance(starts, ninst, b);
   const int64_t s0 = starts[i];
-  elem_pass_body<T, EPT>(ds + i, b - s0, starts[i + 1] - s0, b == s0);
+  elem_pass_body<S, T, EPT>(ds + i, b - s0, starts[i + 1] - s0, b == s0);
 }
 
 __global__ __launch_bounds__(NTHREADS) void scalar_kernel(const PassDesc* __restrict__ d) {
@@ -1338,23 +1352,36 @@ static size_t xform_lds_bytes(int dtype, const PassDesc& h) {
   return (size_t)h.lds_bytes;
 }
 
-template <typename T, int KIND>
+// Pass precisions: 0 = complex FP32 storage and arithmetic, 1 = FP64 both,
+// 2 = complex FP32 storage with FP64 arithmetic.
+template <typename S, typename T, int KIND>
 static void launch_xform(const PassDesc* d_desc, int ept, dim3 grid, size_t lds, int64_t nt, int ib, hipStream_t s) {
   switch (ept) {
-    case 4: xform_pass_kernel<T, KIND, 4><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
-    case 8: xform_pass_kernel<T, KIND, 8><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
-    default: xform_pass_kernel<T, KIND, 16><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
+    case 4: xform_pass_kernel<S, T, KIND, 4><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
+    case 8: xform_pass_kernel<S, T, KIND, 8><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
+    default: xform_pass_kernel<S, T, KIND, 16><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
   }
 }
 
-template <typename T, int KIND>
+template <typename S, typename T, int KIND>
 static void set_lds_attr() {
-  (void)hipFuncSetAttribute((const void*)xform_pass_kernel<T, KIND, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  (void)hipFuncSetAttribute((const void*)xform_pass_kernel<S, T, KIND, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)SOD_MAX_DYN_LDS);
-  (void)hipFuncSetAttribute((const void*)xform_pass_kernel<T, KIND, 8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  (void)hipFuncSetAttribute((const void*)xform_pass_kernel<S, T, KIND, 8>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)SOD_MAX_DYN_LDS);
-  (void)hipFuncSetAttribute((const void*)xform_pass_kernel<T, KIND, 16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  (void)hipFuncSetAttribute((const void*)xform_pass_kernel<S, T, KIND, 16>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)SOD_MAX_DYN_LDS);
+}
+
+template <typename S, typename T>
+static void launch_pass_st(const PassDesc* d_desc, const PassDesc& h, dim3 grid, size_t lds, int64_t nt, int ib,
+                           hipStream_t s) {
+  if (h.nx == 0)
+    elem_pass_kernel<S, T, SOD_ELEM_EPT><<<grid, NTHREADS, 0, s>>>(d_desc, nt, ib);
+  else if (h.xf[0].type == XF_MAT)
+    launch_xform<S, T, XF_MAT>(d_desc, h.ept, grid, lds, nt, ib, s);
+  else
+    launch_xform<S, T, XF_FFT>(d_desc, h.ept, grid, lds, nt, ib, s);
 }
 
 int sod_pass_lds_bytes(int dtype, const PassDesc& h) { return (int)xform_lds_bytes(dtype, h); }
@@ -1362,90 +1389,79 @@ int sod_pass_lds_bytes(int dtype, const PassDesc& h) { return (int)xform_lds_byt
 int sod_launch_pass(int dtype, const PassDesc* d_desc, const PassDesc& h, hipStream_t s) {
   const int ib = h.inherit_block && h.prev_nscalar > 0 ? 1 : 0;
   const int64_t nt = h.ntiles;
-  const dim3 grid((unsigned)(nt + ib)), block(NTHREADS);
-  if (h.nx == 0) {
-    if (dtype)
-      elem_pass_kernel<double, SOD_ELEM_EPT><<<grid, block, 0, s>>>(d_desc, nt, ib);
-    else
-      elem_pass_kernel<float, SOD_ELEM_EPT><<<grid, block, 0, s>>>(d_desc, nt, ib);
-  } else {
-    const size_t lds = xform_lds_bytes(dtype, h);
-    static bool attr_set = false;
-    if (!attr_set) {
-      set_lds_attr<float, XF_FFT>();
-      set_lds_attr<float, XF_MAT>();
-      set_lds_attr<double, XF_FFT>();
-      set_lds_attr<double, XF_MAT>();
-      attr_set = true;
-    }
-    if (lds > SOD_MAX_DYN_LDS) return (int)hipErrorInvalidValue;
-    if ((int64_t)h.tl * h.n > (int64_t)h.ept * NTHREADS) return (int)hipErrorInvalidValue;
-    const bool mat = h.xf[0].type == XF_MAT;
-    if (dtype) {
-      if (mat)
-        launch_xform<double, XF_MAT>(d_desc, h.ept, grid, lds, nt, ib, s);
-      else
-        launch_xform<double, XF_FFT>(d_desc, h.ept, grid, lds, nt, ib, s);
-    } else {
-      if (mat)
-        launch_xform<float, XF_MAT>(d_desc, h.ept, grid, lds, nt, ib, s);
-      else
-        launch_xform<float, XF_FFT>(d_desc, h.ept, grid, lds, nt, ib, s);
-    }
+  const dim3 grid((unsigned)(nt + ib));
+  const size_t lds = h.nx ? xform_lds_bytes(dtype, h) : 0;
+  static bool attr_set = false;
+  if (!attr_set) {
+    set_lds_attr<float, float, XF_FFT>();
+    set_lds_attr<float, float, XF_MAT>();
+    set_lds_attr<double, double, XF_FFT>();
+    set_lds_attr<double, double, XF_MAT>();
+    set_lds_attr<float, double, XF_FFT>();
+    set_lds_attr<float, double, XF_MAT>();
+    attr_set = true;
+  }
+  if (lds > SOD_MAX_DYN_LDS) return (int)hipErrorInvalidValue;
+  if (h.nx && (int64_t)h.tl * h.n > (int64_t)h.ept * NTHREADS) return (int)hipErrorInvalidValue;
+  switch (dtype) {
+    case 0: launch_pass_st<float, float>(d_desc, h, grid, lds, nt, ib, s); break;
+    case 1: launch_pass_st<double, double>(d_desc, h, grid, lds, nt, ib, s); break;
+    case 2: launch_pass_st<float, double>(d_desc, h, grid, lds, nt, ib, s); break;
+    default: return (int)hipErrorInvalidValue;
   }
   return (int)hipGetLastError();
 }
 
-template <typename T, int KIND>
+template <typename S, typename T, int KIND>
 static void launch_xform_batch(const PassDesc* ds, const int64_t* starts, int ninst, int ept, dim3 grid,
                                size_t lds, hipStream_t s) {
   switch (ept) {
-    case 4: xform_batch_kernel<T, KIND, 4><<<grid, NTHREADS, lds, s>>>(ds, starts, ninst); break;
-    case 8: xform_batch_kernel<T, KIND, 8><<<grid, NTHREADS, lds, s>>>(ds, starts, ninst); break;
-    default: xform_batch_kernel<T, KIND, 16><<<grid, NTHREADS, lds, s>>>(ds, starts, ninst); break;
+    case 4: xform_batch_kernel<S, T, KIND, 4><<<grid, NTHREADS, lds, s>>>(ds, starts, ninst); break;
+    case 8: xform_batch_kernel<S, T, KIND, 8><<<grid, NTHREADS, lds, s>>>(ds, starts, ninst); break;
+    default: xform_batch_kernel<S, T, KIND, 16><<<grid, NTHREADS, lds, s>>>(ds, starts, ninst); break;
   }
 }
 
-template <typename T, int KIND>
+template <typename S, typename T, int KIND>
 static void set_lds_attr_batch() {
-  (void)hipFuncSetAttribute((const void*)xform_batch_kernel<T, KIND, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  (void)hipFuncSetAttribute((const void*)xform_batch_kernel<S, T, KIND, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)SOD_MAX_DYN_LDS);
-  (void)hipFuncSetAttribute((const void*)xform_batch_kernel<T, KIND, 8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  (void)hipFuncSetAttribute((const void*)xform_batch_kernel<S, T, KIND, 8>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)SOD_MAX_DYN_LDS);
-  (void)hipFuncSetAttribute((const void*)xform_batch_kernel<T, KIND, 16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  (void)hipFuncSetAttribute((const void*)xform_batch_kernel<S, T, KIND, 16>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)SOD_MAX_DYN_LDS);
+}
+
+template <typename S, typename T>
+static void launch_batch_st(int nx, int xf_type, const PassDesc* d_descs, const int64_t* d_starts, int ninst,
+                            int ept, dim3 grid, size_t lds, hipStream_t s) {
+  if (nx == 0)
+    elem_batch_kernel<S, T, SOD_ELEM_EPT><<<grid, NTHREADS, 0, s>>>(d_descs, d_starts, ninst);
+  else if (xf_type == XF_MAT)
+    launch_xform_batch<S, T, XF_MAT>(d_descs, d_starts, ninst, ept, grid, lds, s);
+  else
+    launch_xform_batch<S, T, XF_FFT>(d_descs, d_starts, ninst, ept, grid, lds, s);
 }
 
 int sod_launch_batch(int dtype, int nx, int xf_type, const PassDesc* d_descs, const int64_t* d_starts,
                      int ninst, int64_t total_tiles, int ept, size_t lds, hipStream_t s) {
   const dim3 grid((unsigned)total_tiles);
-  if (nx == 0) {
-    if (dtype)
-      elem_batch_kernel<double, SOD_ELEM_EPT><<<grid, NTHREADS, 0, s>>>(d_descs, d_starts, ninst);
-    else
-      elem_batch_kernel<float, SOD_ELEM_EPT><<<grid, NTHREADS, 0, s>>>(d_descs, d_starts, ninst);
-    return (int)hipGetLastError();
-  }
   static bool attr_set = false;
   if (!attr_set) {
-    set_lds_attr_batch<float, XF_FFT>();
-    set_lds_attr_batch<float, XF_MAT>();
-    set_lds_attr_batch<double, XF_FFT>();
-    set_lds_attr_batch<double, XF_MAT>();
+    set_lds_attr_batch<float, float, XF_FFT>();
+    set_lds_attr_batch<float, float, XF_MAT>();
+    set_lds_attr_batch<double, double, XF_FFT>();
+    set_lds_attr_batch<double, double, XF_MAT>();
+    set_lds_attr_batch<float, double, XF_FFT>();
+    set_lds_attr_batch<float, double, XF_MAT>();
     attr_set = true;
   }
   if (lds > SOD_MAX_DYN_LDS) return (int)hipErrorInvalidValue;
-  const bool mat = xf_type == XF_MAT;
-  if (dtype) {
-    if (mat)
-      launch_xform_batch<double, XF_MAT>(d_descs, d_starts, ninst, ept, grid, lds, s);
-    else
-      launch_xform_batch<double, XF_FFT>(d_descs, d_starts, ninst, ept, grid, lds, s);
-  } else {
-    if (mat)
-      launch_xform_batch<float, XF_MAT>(d_descs, d_starts, ninst, ept, grid, lds, s);
-    else
-      launch_xform_batch<float, XF_FFT>(d_descs, d_starts, ninst, ept, grid, lds, s);
+  switch (dtype) {
+    case 0: launch_batch_st<float, float>(nx, xf_type, d_descs, d_starts, ninst, ept, grid, lds, s); break;
+    case 1: launch_batch_st<double, double>(nx, xf_type, d_descs, d_starts, ninst, ept, grid, lds, s); break;
+    case 2: launch_batch_st<float, double>(nx, xf_type, d_descs, d_starts, ninst, ept, grid, lds, s); break;
+    default: return (int)hipErrorInvalidValue;
   }
   return (int)hipGetLastError();
 }

@@ -244,7 +244,13 @@ struct so_wave_function {
 
 static void flush_owner(so_wave_function* wf);
 
+// bytes of one stored complex value
 static size_t csize(int dtype) { return dtype == SO_COMPLEX_F64 ? 16 : 8; }
+// storage precision of psi (0 = FP32, 1 = FP64) and the precision of the pass
+// arithmetic (tables, twiddles, GEMM operands, LDS tiles)
+static int store_dt(int dtype) { return dtype == SO_COMPLEX_F64 ? 1 : 0; }
+static int arith_dt(int dtype) { return dtype == SO_COMPLEX_F32 ? 0 : 1; }
+static size_t acsize(int dtype) { return arith_dt(dtype) ? 16 : 8; }
 
 static void wf_check_grids(so_wave_function* wf) {
   for (int a = 0; a < wf->ndim; ++a)
@@ -264,7 +270,8 @@ static std::vector<const double*> wf_weights(so_wave_function* wf) {
 
 static so_wave_function* wf_new(so_context* ctx, int ndim, const int64_t* shape, int dtype) {
   if (ndim < 1 || ndim > MAXD) fail(SO_ERR_UNSUPPORTED, "ndim %d not in 1..%d", ndim, MAXD);
-  if (dtype != SO_COMPLEX_F32 && dtype != SO_COMPLEX_F64) fail(SO_ERR_INVALID, "bad dtype %d", dtype);
+  if (dtype != SO_COMPLEX_F32 && dtype != SO_COMPLEX_F64 && dtype != SO_COMPLEX_F32_F64_ARITH)
+    fail(SO_ERR_INVALID, "bad dtype %d", dtype);
   auto* wf = new so_wave_function();
   wf->ctx = ctx;
   wf->ndim = ndim;
@@ -608,7 +615,7 @@ struct Propagation {
       e.kind = sod::EK_MUL1D;
       e.axis = op->dim;
     }
-    e.table = op->table.get(dt);
+    e.table = op->table.get(arith_dt(dt));
     emit_elem(e);
     wf->possible_norm_change = true;
   }
@@ -857,7 +864,7 @@ CompiledPass* Propagation::compile(const PassB& b, bool immediate) {
     for (int a = axis + 1; a < wf->ndim; ++a) h.inner *= wf->shape[a];
     h.nx = b.nx;
     const int64_t n = h.n;
-    const size_t cs = csize(dt);
+    const size_t cs = acsize(dt);
     bool has_mat = false;
     for (int i = 0; i < b.nx; ++i) {
       const XSpec& x = b.xf[i];
@@ -870,12 +877,12 @@ CompiledPass* Propagation::compile(const PassB& b, bool immediate) {
         d.nrad = (int)rad.size();
         for (int r = 0; r < d.nrad; ++r) d.rad[r] = rad[r];
         d.inverse = x.which;
-        d.tw = fft_twiddles(x.op, dt);
+        d.tw = fft_twiddles(x.op, arith_dt(dt));
         d.scale = 1.0 / std::sqrt((double)n);
       } else {
         has_mat = true;
         int cplx = 0;
-        d.B = gemm_operand(x.op, x.which, dt, &cplx);
+        d.B = gemm_operand(x.op, x.which, arith_dt(dt), &cplx);
         d.cplx = cplx;
         d.kdim = (int)(cplx ? 2 * n : n);
         d.kpad = (d.kdim + 3) / 4 * 4;
@@ -1037,7 +1044,7 @@ CompiledPass* Propagation::compile(const PassB& b, bool immediate) {
 
 void Propagation::run_saver(const SaveAct& s) {
   so_op* sv = s.saver;
-  const int dt = dtype();
+  const int dt = store_dt(dtype());
   if (!sv->d_frames) {
     sv->d_frames = (double*)ctx->alloc(sizeof(double) * sv->per_frame * sv->frames_no);
     HIPCHK(hipMemsetAsync(sv->d_frames, 0, sizeof(double) * sv->per_frame * sv->frames_no, ctx->stream));
@@ -1370,7 +1377,7 @@ extern "C" int so_wave_function_dot(so_wave_function* a, so_wave_function* b, do
     if (a->grids[i].weights != b->grids[i].weights) fail(SO_ERR_SHAPE, "dot: weight arrays differ (wave_function.rs:91)");
   }
   auto w = wf_weights(a);
-  HIPCHK((hipError_t)sod_launch_dot(a->dtype, a->data, b->data, a->ndim, a->shape, w.data(), a->d_partials,
+  HIPCHK((hipError_t)sod_launch_dot(store_dt(a->dtype), a->data, b->data, a->ndim, a->shape, w.data(), a->d_partials,
                                     a->d_counter, a->d_out, a->ctx->stream));
   double r[2];
   HIPCHK(hipMemcpyAsync(r, a->d_out, sizeof(r), hipMemcpyDeviceToHost, a->ctx->stream));
@@ -1387,7 +1394,7 @@ extern "C" int so_wave_function_density(so_wave_function* wf, double* out) {
   NEED(out);
   flush_owner(wf);
   double* st = wf_stage(wf);
-  HIPCHK((hipError_t)sod_launch_density(wf->dtype, wf->data, wf->size, st, 1, wf->ctx->stream));
+  HIPCHK((hipError_t)sod_launch_density(store_dt(wf->dtype), wf->data, wf->size, st, 1, wf->ctx->stream));
   HIPCHK(hipMemcpyAsync(out, st, sizeof(double) * wf->size, hipMemcpyDeviceToHost, wf->ctx->stream));
   wf->ctx->sync();
   API_END
@@ -1403,10 +1410,10 @@ extern "C" int so_wave_function_state_density(so_wave_function* wf, int axis, do
   double* st = wf_stage(wf);
   int64_t nout = wf->size;
   if (wf->ndim == 1) {
-    HIPCHK((hipError_t)sod_launch_density(wf->dtype, wf->data, wf->size, st, 1, wf->ctx->stream));
+    HIPCHK((hipError_t)sod_launch_density(store_dt(wf->dtype), wf->data, wf->size, st, 1, wf->ctx->stream));
   } else {
     auto w = wf_weights(wf);
-    HIPCHK((hipError_t)sod_launch_state_density(wf->dtype, wf->data, wf->ndim, wf->shape, axis, w.data(), st,
+    HIPCHK((hipError_t)sod_launch_state_density(store_dt(wf->dtype), wf->data, wf->ndim, wf->shape, axis, w.data(), st,
                                                 wf->ctx->stream));
     nout = wf->shape[axis];
   }

@@ -72,7 +72,7 @@ def test_fft_norm_roundtrip(fixtures):
 
 
 # ------------------------------------------------------------------ lane transforms
-@pytest.mark.parametrize("dtype,tol", [("f64", 1e-13), ("f32", 3e-6)])
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-13), ("f32", 3e-6), ("f32x", 4e-7)])
 @pytest.mark.parametrize("shape,axis", [((512,), 0), ((6, 768), 1), ((1536, 5), 0), ((12, 64, 18), 1),
                                         ((8, 6, 256), 2), ((256, 3, 4), 0), ((4, 2048), 1), ((2048, 3), 0)])
 def test_fft_lanes_match_numpy(dtype, tol, shape, axis):
@@ -89,7 +89,7 @@ def test_fft_lanes_match_numpy(dtype, tol, shape, axis):
     assert rel(wf.array, a) < tol
 
 
-@pytest.mark.parametrize("dtype,tol", [("f64", 1e-13), ("f32", 3e-6)])
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-13), ("f32", 3e-6), ("f32x", 4e-7)])
 @pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("shape,axis", [((40, 32), 1), ((32, 24), 0), ((3, 5, 4), 1), ((100, 128), 1), ((6, 64, 5), 1)])
 def test_matrix_transformation_matches_numpy(dtype, tol, cplx, shape, axis):
@@ -160,7 +160,7 @@ def test_harmonic_fixture_f32(fixtures):
 
 
 # ------------------------------------------------------------------ workloads vs oracle
-@pytest.mark.parametrize("dtype,tol", [("f64", 1e-10), ("f32", 2e-5)])
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-10), ("f32", 2e-5), ("f32x", 6e-6)])
 def test_harmonic_1d_1000_steps(dtype, tol):
     """1-D harmonic trap, 1024 points, dt = 0.01, 1000 steps with FP64 LeakControl."""
     p, g = W.harmonic_1d(so, dtype=dtype)
@@ -176,7 +176,7 @@ def test_harmonic_1d_1000_steps(dtype, tol):
     assert abs(mean_x - 2.0 * np.cos(1000 * np.arccos(1 - 0.01 ** 2 / 2))) < (1e-9 if dtype == "f64" else 1e-4)
 
 
-@pytest.mark.parametrize("dtype,tol", [("f64", 1e-10), ("f32", 5e-5)])
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-10), ("f32", 5e-5), ("f32x", 3e-6)])
 def test_radial_angular_small_vs_oracle(dtype, tol):
     """The 2-D radial x angular stack at reduced size (256 x 32, 60 steps) vs the oracle,
     including the border loss (loss_checker.rs:54-63)."""
@@ -200,6 +200,20 @@ def test_radial_angular_border_loss_and_leak():
     po.propagate()
     assert rel_l2(p.wave_function().array, po.wave_function().array) < 1e-10
     assert abs(p.wave_function().norm() - po.wave_function().norm()) < 1e-12
+
+
+@pytest.mark.parametrize("dtype,tol", [("f32x", 1e-5), ("f32", 1e-4)])
+def test_radial_angular_full_size_accuracy_1000_steps(dtype, tol):
+    """The 2048 x 128 real-time workload after 1000 steps against the same stack stored in FP64: complex FP32
+    storage with FP64 pass arithmetic stays within 1e-5 relative L2 (FP32 storage rounding only, which does
+    not repeat from step to step); FP32 arithmetic accumulates the same transform rounding every step."""
+    kw = dict(n_r=2048, n_theta=128, steps=1000, dt=0.002, gl=so.gauss_legendre(128))
+    p, _, _ = W.radial_angular(so, dtype=dtype, **kw)
+    p.steps(1000)
+    q, _, _ = W.radial_angular(so, dtype="f64", **kw)
+    q.steps(1000)
+    err = rel_l2(p.wave_function().array, q.wave_function().array)
+    assert err < tol, err
 
 
 def test_graphs_replay_matches_eager():
@@ -282,7 +296,7 @@ def test_savers_and_loss_saver(tmp_path, monkeypatch):
     assert saved.shape == (16, 8) and np.array_equal(saved, d1)
 
 
-@pytest.mark.parametrize("dtype,tol", [("f64", 1e-10), ("f32", 5e-5)])
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-10), ("f32", 5e-5), ("f32x", 3e-6)])
 def test_cartesian_3d_small_vs_oracle(dtype, tol):
     """The 3-D Cartesian stack (per-axis FFT lanes, strided and contiguous) at 32^3, 40 steps."""
     kw = dict(n=32, steps=40, dt=0.01)
