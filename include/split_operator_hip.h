@@ -43,6 +43,9 @@ enum {
 /* device storage of psi: complex FP32, complex FP64, or complex FP32 storage with FP64 arithmetic in the passes
  * (tables, twiddles, Legendre matrices and LDS tiles in FP64; HBM traffic of FP32) */
 enum { SO_COMPLEX_F32 = 0, SO_COMPLEX_F64 = 1, SO_COMPLEX_F32_F64_ARITH = 2 };
+/* lane GEMM of matrix transformations in FP32-arithmetic passes: FP32 MFMA, or each FP32 operand split into two
+ * BF16 parts and the product formed as hi.hi + hi.lo + lo.hi on BF16 MFMA (FP32 sums, ~2^-16 relative; K <= 128) */
+enum { SO_GEMM_FP32 = 0, SO_GEMM_SPLIT_BF16X3 = 1 };
 enum { SO_APPLY_NONE = 0, SO_APPLY_FIRST_HALF = 1, SO_APPLY_SECOND_HALF = 2 }; /* control.rs:19-26 */
 enum { SO_ORDER_NORMAL = 0, SO_ORDER_INVERSE_FIRST = 1 };        /* transformation.rs:13-17 */
 
@@ -166,6 +169,7 @@ int so_operation_stack_destroy(so_operation_stack *s);
 int so_propagation_create(so_context *ctx, so_propagation **out);                        /* Default */
 int so_propagation_set_wave_function(so_propagation *p, so_wave_function *wf);           /* :91-93 (copies) */
 int so_propagation_set_time_grid(so_propagation *p, double step, int64_t step_no, int im_time); /* :96-98 */
+int so_propagation_set_gemm_mode(so_propagation *p, int mode); /* no crate counterpart; SO_GEMM_FP32 by default */
 int so_propagation_set_operation_stack(so_propagation *p, so_operation_stack *s);        /* :101-103 (moves) */
 int so_propagation_wave_function(so_propagation *p, so_wave_function **borrowed);        /* :111-113 */
 int so_propagation_propagate(so_propagation *p);                                         /* :185-190 */

@@ -484,6 +484,11 @@ class Propagation:
             self.set_operation_stack(self._pending)
             self._pending = None
 
+    def set_gemm_mode(self, mode):
+        """Lane GEMM of FP32-arithmetic passes: "fp32" (default) or "bf16x3" (split-BF16 x3 on BF16 MFMA)."""
+        self._ready()
+        call("so_propagation_set_gemm_mode", self._h, {"fp32": 0, "bf16x3": 1}[mode])
+
     def step_once(self):
         self.steps(1)
 

@@ -72,7 +72,9 @@ struct XformDesc {
   int32_t kpad;            // MAT: K rounded up to 4
   int32_t ncols;           // MAT: N (n or 2n)
   int32_t npad;            // MAT: N rounded up to 16
-  int32_t pad2;
+  int32_t bf16x3;          // MAT, FP32 arithmetic: product as split-BF16 x3 on BF16 MFMA
+  const void* Bh;          // bf16x3: BF16 high parts of B, [32][npad][4] (element (k, c) at ((k/4) npad + c) 4 + k%4)
+  const void* Bl;          // bf16x3: BF16 low parts (B - high), same layout
 };
 
 enum ScalarKind : int32_t {

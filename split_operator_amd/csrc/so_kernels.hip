@@ -389,8 +389,43 @@ __device__ __forceinline__ bool mat_fast(const XformDesc& x, int tl) {
 
 // B fragments of the column blocks round `v0` of this wave (v0 = 0 at
 // kernel start, so their latency overlaps the psi loads)
+// split-BF16x3 fragments: for 16-k block c of column block u, br.b[u][4c..4c+1]
+// hold the four BF16 high parts of B[16c + 4 lk + i][col] (i = 0..3) and
+// br.b[u][4c+2..4c+3] the four low parts
+__device__ __forceinline__ void mat_load_b_bf16(const XformDesc& x, int tid, int v0, BRegs<float>& br) {
+  const int ctiles = x.npad / 16;
+  const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const MatPlan pl = mat_plan(ctiles, wave);
+  const uint2* Bh = (const uint2*)x.Bh;
+  const uint2* Bl = (const uint2*)x.Bl;
+#pragma unroll
+  for (int u = 0; u < BFast<float>::TPW; ++u) {
+    const int ct = pl.col0 + pl.cstep * (v0 + u);
+    const bool has = ct < ctiles;
+    const int col = (has ? ct : 0) * 16 + lr;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      uint2 h = {0u, 0u}, l = {0u, 0u};
+      if (has && 16 * c < x.kpad) {
+        h = gld(&Bh[(4 * c + lk) * x.npad + col]);
+        l = gld(&Bl[(4 * c + lk) * x.npad + col]);
+      }
+      br.b[u][4 * c + 0] = __builtin_bit_cast(float, h.x);
+      br.b[u][4 * c + 1] = __builtin_bit_cast(float, h.y);
+      br.b[u][4 * c + 2] = __builtin_bit_cast(float, l.x);
+      br.b[u][4 * c + 3] = __builtin_bit_cast(float, l.y);
+    }
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void mat_load_b(const XformDesc& x, int tid, int v0, BRegs<T>& br) {
+  if constexpr (sizeof(T) == 4) {
+    if (x.bf16x3) {
+      mat_load_b_bf16(x, tid, v0, br);
+      return;
+    }
+  }
   const int ctiles = x.npad / 16;
   const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
   const MatPlan pl = mat_plan(ctiles, wave);
@@ -427,6 +462,73 @@ __device__ __forceinline__ void mat_store_tile(T* Cm, const ACC& acc, int rt, in
         lst(&Cm[r * ld * 2 + col], (T)acc[j]);
       else
         lst(&Cm[((r >> 1) * ld + col) * 2 + (r & 1)], (T)acc[j]);
+    }
+  }
+}
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16 bits, round to nearest even (finite inputs)
+__device__ __forceinline__ uint32_t bf16_bits(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+// Split-BF16x3 product for NKB blocks of 16 k-steps (kpad <= 16 NKB): each
+// FP32 operand a is split into hi = bf16(a) and lo = bf16(a - hi), and
+// A B ~ Ahi Bhi + Ahi Blo + Alo Bhi on 16x16x16 BF16 MFMA with FP32 sums (the
+// dropped Alo Blo term is ~2^-16 relative). Lane L supplies A[L%16][4(L/16)+i]
+// and B[4(L/16)+i][L%16]; the output layout is the FP32 instruction's.
+template <int NKB>
+__device__ __forceinline__ void mat_bf16x3_run(const float* A, float* Cm, int tl, int ld, const XformDesc& x, int tid,
+                                               BRegs<float>& br) {
+  using M = Mfma<float>;
+  constexpr int TPW = BFast<float>::TPW;
+  const int rows = x.cplx ? tl : 2 * tl;
+  const int rtiles = (rows + 15) / 16;
+  const int ctiles = x.npad / 16;
+  const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const int kdim = x.kdim, ncols = x.ncols;
+  const MatPlan pl = mat_plan(ctiles, wave);
+  const int astride = x.cplx ? 1 : 2;
+  for (int v0 = 0; pl.col0 + pl.cstep * v0 < ctiles; v0 += TPW) {
+    if (v0 > 0) mat_load_b_bf16(x, tid, v0, br);
+    int ct[TPW];
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) ct[u] = pl.col0 + pl.cstep * (v0 + u);
+    for (int rt = pl.row0; rt < rtiles; rt += pl.rstep) {
+      const int row = rt * 16 + lr;
+      const bool rv = row < rows;
+      const int rc = rv ? row : 0;
+      const int abase = x.cplx ? rc * ld * 2 : (rc >> 1) * ld * 2 + (rc & 1);
+      M::acc_t acc[TPW];
+#pragma unroll
+      for (int u = 0; u < TPW; ++u) acc[u] = M::acc_t{0, 0, 0, 0};
+#pragma unroll
+      for (int c = 0; c < NKB; ++c) {
+        bf16x4_t ah, al;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int k = 16 * c + 4 * lk + i;
+          const float v = lld(&A[abase + (k < kdim ? k : 0) * astride]);
+          const float a = (rv && k < kdim) ? v : 0.0f;
+          const uint32_t h = bf16_bits(a);
+          ah[i] = (short)h;
+          al[i] = (short)bf16_bits(a - __builtin_bit_cast(float, h << 16));
+        }
+#pragma unroll
+        for (int u = 0; u < TPW; ++u) {
+          const bf16x4_t bh = __builtin_bit_cast(bf16x4_t, make_float2(br.b[u][4 * c], br.b[u][4 * c + 1]));
+          const bf16x4_t bl = __builtin_bit_cast(bf16x4_t, make_float2(br.b[u][4 * c + 2], br.b[u][4 * c + 3]));
+          acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc[u], 0, 0, 0);
+          acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc[u], 0, 0, 0);
+          acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc[u], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < TPW; ++u)
+        if (ct[u] < ctiles) mat_store_tile<float, float2>(Cm, acc[u], rt, ct[u], rows, ncols, ld, x.cplx, lr, lk);
     }
   }
 }
@@ -503,6 +605,18 @@ __device__ __forceinline__ void mat_lds(const T2* in, T2* out, int tl, int ld, c
   const int lk = lane >> 4;
   const int kdim = x.kdim, kpad = x.kpad, npad = x.npad, ncols = x.ncols;
   if (fast) {
+    if constexpr (sizeof(T) == 4) {
+      if (x.bf16x3) {
+        const int nkb = (kpad + 15) / 16;
+        if (nkb <= 2)
+          mat_bf16x3_run<2>(A, Cm, tl, ld, x, tid, br);
+        else if (nkb <= 4)
+          mat_bf16x3_run<4>(A, Cm, tl, ld, x, tid, br);
+        else
+          mat_bf16x3_run<8>(A, Cm, tl, ld, x, tid, br);
+        return;
+      }
+    }
     // chunks of AC k-steps that cover kpad: straight-line MFMA chains
     const int nch = (kpad / 4 + 7) / 8;
     if (nch <= 1)

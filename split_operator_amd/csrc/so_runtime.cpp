@@ -202,6 +202,8 @@ struct so_op {
   DevTable twiddles;
   std::vector<double> mat_t, mat_tinv;  // complex n x n
   void* gemm_b[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [which][dtype]
+  void* gemm_bh[2] = {nullptr, nullptr};  // [which]: split-BF16 high parts of the FP32 operand
+  void* gemm_bl[2] = {nullptr, nullptr};  // [which]: low parts
   int gemm_cplx[2] = {0, 0};
   // controls
   LossCheckerH chk;
@@ -444,6 +446,7 @@ struct Propagation {
   // member of a batch: tiles are sized for throughput (the batch as a whole
   // supplies the workgroups) rather than for filling the GPU on their own
   bool batch_mode = false;
+  int gemm_mode = SO_GEMM_FP32;  // lane GEMM of FP32-arithmetic passes
   struct GraphRec {
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
@@ -787,6 +790,59 @@ static void* gemm_operand(so_op* op, int which, int dtype, int* cplx_out) {
   return d;
 }
 
+static uint16_t bf16_rne_host(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// Split-BF16 parts of the FP32 GEMM operand for K <= 128: hi = bf16(b),
+// lo = bf16(b - hi), each laid out [32][npad][4] (element (k, c) at
+// ((k/4) npad + c) 4 + k%4) so a lane reads its four k values in one 8-byte load.
+static void gemm_operand_bf16(so_op* op, int which, void** hi, void** lo) {
+  if (!op->gemm_bh[which]) {
+    int cplx = 0;
+    const float* dummy = (const float*)gemm_operand(op, which, 0, &cplx);
+    (void)dummy;
+    const std::vector<double>& m = which ? op->mat_tinv : op->mat_t;
+    const int64_t n = op->n;
+    const int64_t K = cplx ? 2 * n : n, N = K;
+    const int64_t npad = (N + 15) / 16 * 16;
+    std::vector<double> B((size_t)(128 * npad), 0.0);
+    for (int64_t r = 0; r < n; ++r)
+      for (int64_t k = 0; k < n; ++k) {
+        const double mr = m[2 * (r * n + k)], mi = m[2 * (r * n + k) + 1];
+        if (!cplx) {
+          B[k * npad + r] = mr;
+        } else {
+          B[(2 * k) * npad + 2 * r] = mr;
+          B[(2 * k + 1) * npad + 2 * r] = -mi;
+          B[(2 * k) * npad + 2 * r + 1] = mi;
+          B[(2 * k + 1) * npad + 2 * r + 1] = mr;
+        }
+      }
+    std::vector<uint16_t> h((size_t)(128 * npad)), l((size_t)(128 * npad));
+    for (int64_t k = 0; k < 128; ++k)
+      for (int64_t c = 0; c < npad; ++c) {
+        const double b = B[k * npad + c];
+        const uint16_t bh = bf16_rne_host((float)b);
+        uint32_t bits = (uint32_t)bh << 16;
+        float fh;
+        memcpy(&fh, &bits, 4);
+        const size_t at = (size_t)(((k / 4) * npad + c) * 4 + k % 4);
+        h[at] = bh;
+        l[at] = bf16_rne_host((float)(b - (double)fh));
+      }
+    op->gemm_bh[which] = op->ctx->alloc(2 * h.size());
+    op->gemm_bl[which] = op->ctx->alloc(2 * l.size());
+    HIPCHK(hipMemcpy(op->gemm_bh[which], h.data(), 2 * h.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(op->gemm_bl[which], l.data(), 2 * l.size(), hipMemcpyHostToDevice));
+  }
+  *hi = op->gemm_bh[which];
+  *lo = op->gemm_bl[which];
+}
+
 // Twiddles grouped per Stockham pass, in the pass order of radix_plan(n): for
 // pass p (Ls = product of the earlier radices, radix R) the entries
 // W_n^(r k n/(Ls R)), r = 1..R-1, k = 0..Ls-1, stored at (r-1) Ls + k. The
@@ -888,6 +944,13 @@ CompiledPass* Propagation::compile(const PassB& b, bool immediate) {
         d.kpad = (d.kdim + 3) / 4 * 4;
         d.ncols = d.kdim;
         d.npad = (d.ncols + 15) / 16 * 16;
+        if (gemm_mode == SO_GEMM_SPLIT_BF16X3 && arith_dt(dt) == 0 && d.kpad <= 128) {
+          void *bh = nullptr, *bl = nullptr;
+          gemm_operand_bf16(x.op, x.which, &bh, &bl);
+          d.bf16x3 = 1;
+          d.Bh = bh;
+          d.Bl = bl;
+        }
       }
     }
     // tile choice
@@ -1974,8 +2037,11 @@ static void op_free(so_op* o) {
   so_context* c = o->ctx;
   o->table.invalidate();
   o->twiddles.invalidate();
-  for (int w = 0; w < 2; ++w)
+  for (int w = 0; w < 2; ++w) {
     for (int d = 0; d < 2; ++d) c->release(o->gemm_b[w][d]);
+    c->release(o->gemm_bh[w]);
+    c->release(o->gemm_bl[w]);
+  }
   c->release(o->held.d_weights);
   c->release(o->d_frames);
   c->release(o->chk.d_hist);
@@ -2123,6 +2189,17 @@ extern "C" int so_propagation_set_time_grid(so_propagation* pr, double step, int
   pr->p.step = step;
   pr->p.step_no = step_no;
   pr->p.im_time = im_time != 0;
+  API_END
+}
+
+extern "C" int so_propagation_set_gemm_mode(so_propagation* pr, int mode) {
+  API_BEGIN
+  NEED(pr);
+  if (mode != SO_GEMM_FP32 && mode != SO_GEMM_SPLIT_BF16X3) fail(SO_ERR_INVALID, "unknown GEMM mode %d", mode);
+  if (pr->p.gemm_mode != mode) {
+    prop_reset_compiled(pr->p);
+    pr->p.gemm_mode = mode;
+  }
   API_END
 }
 

@@ -276,7 +276,8 @@ class LearnedWarmStart:
 
     def relax(self, params, horizon, e_ref=None, tol=1e-8, psi0s=None, check_every=256, dtype="f64"):
         """Batched FP64 imaginary time from the cold start (psi0s None) or the given states. Records the decay
-        energy of every instance at each check; with e_ref, an instance retires once |E - E_ref| <= tol |E_ref|.
+        energy of every instance at each check; with e_ref, an instance retires once E - E_ref <= tol |E_ref|
+        (imaginary time lowers the energy monotonically, so this is "within tol of the reference, or below it").
         Returns per-instance dicts and the wall time of the stepping."""
         insts = []
         for i, pp in enumerate(params):
@@ -303,7 +304,7 @@ class LearnedWarmStart:
                 e = W.decay_energy(tail[0], tail[1], self.dt)
                 it["hist"].append((done, now, e))
                 it["steps"] = done
-                if e_ref is not None and abs(e - e_ref[i]) <= tol * abs(e_ref[i]):
+                if e_ref is not None and e - e_ref[i] <= tol * abs(e_ref[i]):
                     it["met"] = (done, now)
                     active.remove(i)
                     b.remove(it["prop"])

@@ -143,7 +143,8 @@ def decay_energy(l_prev, l_cur, dt):
 
 class GroundStateBatch:
     """Imaginary-time relaxation of many radial x angular instances at once (uneven grids, each with
-    its own seeded potential). Every instance runs the stack [LeakControl (FP64 renormalisation each
+    its own seeded potential; `gemm` selects the lane GEMM of FP32 passes). Every instance runs the stack
+    [LeakControl (FP64 renormalisation each
     step, loss saved every step), V/2, Legendre, Rot/2, FFT_r, T]; the per-step energy is the decay
     estimator of consecutive LeakControl losses. An instance retires from the batch once its relative
     energy change per step falls below `tol`.
@@ -151,7 +152,7 @@ class GroundStateBatch:
     `B` must provide Batch (the HIP backend); `batched=False` steps the same instances one by one."""
 
     def __init__(self, B, count=64, seed0=4000, dt=0.05, max_steps=40000, tol=1e-10, dtype="f64",
-                 sizes=None, batched=True, r0=8.0, sigma=1.0):
+                 sizes=None, batched=True, r0=8.0, sigma=1.0, gemm=None):
         self.B, self.dt, self.max_steps, self.tol = B, dt, max_steps, tol
         self.inst = []
         for i in range(count):
@@ -162,6 +163,8 @@ class GroundStateBatch:
             p, grids, prm = radial_angular(B, n_r=n_r, n_theta=n_t, dt=dt, steps=max_steps, im_time=True,
                                            border=False, leak=True, dtype=dtype, leak_frames=2 * max_steps,
                                            params=prm, r0=r0, sigma=sigma)
+            if gemm is not None:
+                p.set_gemm_mode(gemm)
             self.inst.append({"prop": p, "n_r": n_r, "n_theta": n_t, "params": prm, "steps": 0,
                               "energy": float("nan"), "converged": False, "de": float("nan")})
         self.batched = batched

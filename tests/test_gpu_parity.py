@@ -216,6 +216,66 @@ def test_radial_angular_full_size_accuracy_1000_steps(dtype, tol):
     assert err < tol, err
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("shape", [(100, 128), (64, 32), (40, 64)])
+def test_matrix_transformation_split_bf16x3_single(cplx, shape):
+    """One lane product with the split-BF16 x3 GEMM against numpy: operands carry ~2^-17 relative
+    representation error and the dropped lo.lo term ~2^-16, so a product is within 1e-4 relative (max norm)."""
+    n = shape[1]
+    if cplx and 2 * n > 128:
+        pytest.skip("split-BF16 x3 covers K <= 128")
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    m = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cplx else 0)
+    g0 = so.Grid.new_linear_continuos("x", -1.0, 1.0, shape[0], 0)
+    g1 = so.Grid.new_linear_countable("y", 0.0, float(n - 1), n, 1)
+    gl = so.Grid.new_linear_countable("yl", 0.0, float(n - 1), n, 1)
+    p = so.Propagation()
+    p.set_wave_function(so.WaveFunction(a, [g0, g1], dtype="f32"))
+    p.set_time_grid(0.1, 1, False)
+    p.add_transformation(so.matrix_transformation(g1, gl, m, np.linalg.inv(m)))
+    p.set_gemm_mode("bf16x3")
+    p.steps(1)
+    ref = a @ m.T
+    err = rel(p.wave_function().array, ref)
+    assert err < 1e-4, err
+
+
+def test_radial_angular_split_bf16x3_vs_oracle():
+    """The Legendre products as split-BF16 x3 (hi.hi + hi.lo + lo.hi, FP32 sums): each product carries
+    ~1e-5 relative error, part of it (the representation of the matrices) the same every step, so after
+    60 steps the state is within 5e-4 of the f64 oracle (measured 2.0e-4) and within 1e-3 of the FP32
+    path (a wrong split or operand layout gives O(1) errors)."""
+    kw = dict(n_r=256, n_theta=32, steps=60, dt=0.002)
+    p, _, _ = W.radial_angular(so, dtype="f32", **kw)
+    p.set_gemm_mode("bf16x3")
+    p.propagate()
+    po, _, _ = W.radial_angular(O, **kw)
+    po.propagate()
+    err = rel_l2(p.wave_function().array, po.wave_function().array)
+    assert err < 5e-4, err
+    q, _, _ = W.radial_angular(so, dtype="f32", **kw)
+    q.propagate()
+    assert rel_l2(p.wave_function().array, q.wave_function().array) < 1e-3
+
+
+@pytest.mark.parametrize("n_theta", [32, 64, 128])
+def test_split_bf16x3_imaginary_time_energy(n_theta):
+    """Imaginary time with split-BF16 x3 Legendre products: the ~1e-5 error of the Legendre pair couples
+    l-channels whose centrifugal energies near r = 0.5 are large, so the decay-estimator energy after 400
+    steps agrees with FP64 storage to 3e-3 relative (measured 3e-4 to 1e-3)."""
+    kw = dict(n_r=512, n_theta=n_theta, dt=0.05, steps=400, im_time=True, border=False, leak=True, leak_frames=800)
+    e = []
+    for dtype, gemm in (("f32", "bf16x3"), ("f64", None)):
+        p, _, _ = W.radial_angular(so, dtype=dtype, **kw)
+        if gemm:
+            p.set_gemm_mode(gemm)
+        p.steps(400)
+        t = p.leak.loss_tail(2)
+        e.append(W.decay_energy(t[0], t[1], 0.05))
+    assert abs(e[0] - e[1]) < 3e-3 * abs(e[1]), e
+
+
 def test_graphs_replay_matches_eager():
     """hipGraph replay of steady-state steps gives the same state as eager launches."""
     out = []
