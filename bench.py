@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--c5-warm-steps", type=int, default=10240, help="steps of the warm-started training solutions")
     ap.add_argument("--c5-eval", type=int, default=64, help="held-out potentials of the warm-start evaluation")
     ap.add_argument("--c5-updates", type=int, default=4000, help="optimiser steps per fit")
+    ap.add_argument("--no-autotune", action="store_true", help="tile sizes from the fixed rule instead of timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy a MIB-MiB buffer 3 times (a plain copy kernel of known traffic for "
                          "scaling hardware byte counters), then run the benchmark")
@@ -69,11 +70,13 @@ def pass_record(t):
     return rec
 
 
-def cartesian_256(so, W, steps, kernel_iters):
+def cartesian_256(so, W, steps, kernel_iters, autotune=True):
     """The 256^3 Cartesian workload (complex FP32): throughput and per-pass rates."""
     import time
 
     p, _ = W.cartesian_3d(so, n=256, steps=steps, dtype="f32")
+    if autotune:
+        p.set_autotune(True)
     p.steps(5)
     so.synchronize()
     t0 = time.perf_counter()
@@ -271,6 +274,8 @@ def main():
     p, (radial, polar), (c6, c12, a) = W.radial_angular(
         so, n_r=N_R, n_theta=N_THETA, dt=DT, steps=args.steps, seed=seed, dtype="f32",
         gl=so.gauss_legendre(N_THETA))
+    if not args.no_autotune:
+        p.set_autotune(True)
     p.steps(args.warmup)
     so.synchronize()
     torch.cuda.synchronize()
@@ -346,7 +351,7 @@ def main():
 
     extra = {}
     if args.c3_steps > 0:
-        extra["cartesian_3d_256"] = cartesian_256(so, W, args.c3_steps, args.kernel_iters)
+        extra["cartesian_3d_256"] = cartesian_256(so, W, args.c3_steps, args.kernel_iters, not args.no_autotune)
     if args.c4_instances > 0:
         extra["ground_states_batched"] = ground_states(so, W, args.c4_instances, args.c4_max_steps,
                                                        args.c4_f32_steps, args.c4_solo)

@@ -170,6 +170,10 @@ int so_propagation_create(so_context *ctx, so_propagation **out);               
 int so_propagation_set_wave_function(so_propagation *p, so_wave_function *wf);           /* :91-93 (copies) */
 int so_propagation_set_time_grid(so_propagation *p, double step, int64_t step_no, int im_time); /* :96-98 */
 int so_propagation_set_gemm_mode(so_propagation *p, int mode); /* no crate counterpart; SO_GEMM_FP32 by default */
+/* no crate counterpart: choose tile sizes of transform passes by timing candidates on a scratch copy of psi when a pass
+ * is first compiled (choices cached per pass shape; SO_TUNE_CACHE names a file that keeps them; SO_AUTOTUNE=1 turns it
+ * on by default) */
+int so_propagation_set_autotune(so_propagation *p, int enable);
 int so_propagation_set_operation_stack(so_propagation *p, so_operation_stack *s);        /* :101-103 (moves) */
 int so_propagation_wave_function(so_propagation *p, so_wave_function **borrowed);        /* :111-113 */
 int so_propagation_propagate(so_propagation *p);                                         /* :185-190 */

@@ -489,6 +489,11 @@ class Propagation:
         self._ready()
         call("so_propagation_set_gemm_mode", self._h, {"fp32": 0, "bf16x3": 1}[mode])
 
+    def set_autotune(self, enable=True):
+        """Choose transform-pass tile sizes by timing candidates when a pass is first compiled."""
+        self._ready()
+        call("so_propagation_set_autotune", self._h, 1 if enable else 0)
+
     def step_once(self):
         self.steps(1)
 

@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -730,6 +731,10 @@ struct Propagation {
 
   // ----------------------------------------------------------- compile
   CompiledPass* compile(const PassB& b, bool immediate);
+  std::unique_ptr<CompiledPass> build(const PassB& b, bool immediate, int force_tl);
+  int tune_tile(const PassB& b, bool immediate);
+  bool autotune = std::getenv("SO_AUTOTUNE") != nullptr && std::getenv("SO_AUTOTUNE")[0] == '1';
+  void* tune_scratch = nullptr;
   void launch(const Launch& L);
   void launch_pass(CompiledPass* cp);
   void run_saver(const SaveAct& s);
@@ -878,7 +883,6 @@ static int64_t lds_pitch(bool mat, int64_t n, int64_t tl) {
 }
 
 CompiledPass* Propagation::compile(const PassB& b, bool immediate) {
-  const bool scalar_only = b.nx == 0 && b.st[0].empty();
   std::string key = pass_key(b);
   key_add(key, &pending_prog, sizeof(void*));
   key.push_back(immediate ? 1 : 0);
@@ -888,6 +892,18 @@ CompiledPass* Propagation::compile(const PassB& b, bool immediate) {
     pending_prog = (c->h.has_norm && c->h.fin_mode == sod::FIN_DEFERRED) ? c : nullptr;
     return c;
   }
+  const int tl = (autotune && b.nx > 0) ? tune_tile(b, immediate) : 0;
+  std::unique_ptr<CompiledPass> cp = build(b, immediate, tl);
+  CompiledPass* raw = cp.get();
+  cache.emplace(std::move(key), std::move(cp));
+  pending_prog = (raw->h.has_norm && raw->h.fin_mode == sod::FIN_DEFERRED) ? raw : nullptr;
+  return raw;
+}
+
+// The descriptor of pass b with tile size force_tl (0: the rule below). Pure:
+// no cache or pending-program bookkeeping.
+std::unique_ptr<CompiledPass> Propagation::build(const PassB& b, bool immediate, int force_tl) {
+  const bool scalar_only = b.nx == 0 && b.st[0].empty();
   auto cp = std::make_unique<CompiledPass>();
   sod::PassDesc& h = cp->h;
   memset(&h, 0, sizeof(h));
@@ -979,6 +995,11 @@ CompiledPass* Propagation::compile(const PassB& b, bool immediate) {
       // workgroups resident per CU, so one tile's memory phase overlaps
       // another's transform; halve further while fewer than 512 tiles
       while (tl > min_tl && ((!batch_mode && tiles_for(tl) < 512) || (int64_t)tl * n > 2048)) tl /= 2;
+      if (force_tl > 0) {
+        if (force_tl > sod::MAX_TL || force_tl > std::max<int64_t>(lanes_avail, 1) || (mat && force_tl < 8))
+          fail(SO_ERR_UNSUPPORTED, "tile of %d lanes does not fit this pass", force_tl);
+        tl = force_tl;
+      }
       if ((int64_t)tl * n > (int64_t)sod::FFT_EPT * sod::NTHREADS || lds_for(tl) > SOD_MAX_DYN_LDS)
         fail(SO_ERR_UNSUPPORTED, "lane length %lld exceeds the LDS tile limit for this dtype", (long long)n);
       h.ld = (int)lds_pitch(mat, n, tl);
@@ -1099,10 +1120,109 @@ CompiledPass* Propagation::compile(const PassB& b, bool immediate) {
                         : 0;
   cp->d = (sod::PassDesc*)ctx->alloc(sizeof(sod::PassDesc));
   HIPCHK(hipMemcpy(cp->d, &h, sizeof(sod::PassDesc), hipMemcpyHostToDevice));
-  CompiledPass* raw = cp.get();
-  cache.emplace(std::move(key), std::move(cp));
-  pending_prog = (raw->h.has_norm && raw->h.fin_mode == sod::FIN_DEFERRED) ? raw : nullptr;
-  return raw;
+  return cp;
+}
+
+// ------------------------------------------------------------ tile autotuner
+// Tile sizes chosen by measurement: for a transform pass, tile sizes from a
+// quarter to four times the rule's choice (those that fit) are each launched
+// on a scratch copy of psi with the scalar bookkeeping switched off, and the
+// fastest median of 7 launches wins. Choices are kept per pass shape for the
+// process (and in the file named by SO_TUNE_CACHE, if set).
+static std::map<std::string, int>& tune_table() {
+  static std::map<std::string, int> t;
+  static bool loaded = false;
+  if (!loaded) {
+    loaded = true;
+    if (const char* f = std::getenv("SO_TUNE_CACHE")) {
+      if (FILE* fp = std::fopen(f, "r")) {
+        char k[512];
+        int v;
+        while (std::fscanf(fp, "%511s %d", k, &v) == 2) t[k] = v;
+        std::fclose(fp);
+      }
+    }
+  }
+  return t;
+}
+
+static void tune_store(const std::string& k, int v) {
+  tune_table()[k] = v;
+  if (const char* f = std::getenv("SO_TUNE_CACHE")) {
+    if (FILE* fp = std::fopen(f, "a")) {
+      std::fprintf(fp, "%s %d\n", k.c_str(), v);
+      std::fclose(fp);
+    }
+  }
+}
+
+static void release_pass(so_context* ctx, CompiledPass* cp) {
+  ctx->release(cp->d);
+  ctx->release(cp->h.partials);
+  ctx->release(cp->h.counter);
+}
+
+int Propagation::tune_tile(const PassB& b, bool immediate) {
+  // shape key: precision, batch mode, axis geometry, transform kinds and stage sizes
+  char kb[256];
+  const int axis = b.xf[0].axis;
+  int64_t outer = 1, inner = 1;
+  for (int a = 0; a < axis; ++a) outer *= wf->shape[a];
+  for (int a = axis + 1; a < wf->ndim; ++a) inner *= wf->shape[a];
+  std::snprintf(kb, sizeof(kb), "d%d-b%d-g%d-n%lld-o%lld-i%lld-x%d-%d-%d-s%zu-%zu-%zu", dtype(), batch_mode ? 1 : 0,
+                gemm_mode, (long long)wf->shape[axis], (long long)outer, (long long)inner, b.nx, b.xf[0].type,
+                b.nx > 1 ? b.xf[1].type : 0, b.st[0].size(), b.st[1].size(), b.st[2].size());
+  const std::string key = kb;
+  auto& tt = tune_table();
+  auto it = tt.find(key);
+  if (it != tt.end()) return it->second;
+  std::unique_ptr<CompiledPass> base = build(b, immediate, 0);
+  const int tl0 = base->h.tl;
+  release_pass(ctx, base.get());
+  if (!tune_scratch) tune_scratch = ctx->alloc(csize(dtype()) * wf->size);
+  HIPCHK(hipMemcpyAsync(tune_scratch, wf->data, csize(dtype()) * wf->size, hipMemcpyDeviceToDevice, ctx->stream));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int best = tl0;
+  float best_ms = 1e30f;
+  for (int t = std::max(1, tl0 / 4); t <= tl0 * 4; t *= 2) {
+    std::unique_ptr<CompiledPass> c;
+    try {
+      c = build(b, immediate, t);
+    } catch (const SoError&) {
+      continue;  // does not fit
+    }
+    sod::PassDesc h = c->h;
+    h.psi = tune_scratch;
+    h.has_norm = 0;
+    h.nscalar = 0;
+    h.fin_mode = sod::FIN_NONE;
+    h.prev_nscalar = 0;
+    h.inherit_block = 0;
+    HIPCHK(hipMemcpy(c->d, &h, sizeof(sod::PassDesc), hipMemcpyHostToDevice));
+    std::vector<float> ms;
+    for (int r = 0; r < 9; ++r) {
+      HIPCHK(hipEventRecord(e0, ctx->stream));
+      HIPCHK((hipError_t)sod_launch_pass(dtype(), c->d, h, ctx->stream));
+      HIPCHK(hipEventRecord(e1, ctx->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float m = 0.f;
+      HIPCHK(hipEventElapsedTime(&m, e0, e1));
+      if (r >= 2) ms.push_back(m);
+    }
+    std::sort(ms.begin(), ms.end());
+    const float med = ms[ms.size() / 2];
+    if (med < best_ms) {
+      best_ms = med;
+      best = t;
+    }
+    release_pass(ctx, c.get());
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  tune_store(key, best);
+  return best;
 }
 
 void Propagation::run_saver(const SaveAct& s) {
@@ -1178,6 +1298,7 @@ Propagation::~Propagation() {
     ctx->release(kv.second->h.counter);
   }
   for (auto& kv : scale_slots) ctx->free_slot(kv.second);
+  ctx->release(tune_scratch);
 }
 
 static std::string launches_key(const std::vector<Launch>& v) {
@@ -2161,6 +2282,8 @@ static void prop_reset_compiled(Propagation& p) {
   p.emitted.clear();
   p.pending_prog = nullptr;
   p.last_closed_key.clear();
+  p.ctx->release(p.tune_scratch);
+  p.tune_scratch = nullptr;
 }
 
 extern "C" int so_propagation_set_wave_function(so_propagation* pr, so_wave_function* src) {
@@ -2189,6 +2312,16 @@ extern "C" int so_propagation_set_time_grid(so_propagation* pr, double step, int
   pr->p.step = step;
   pr->p.step_no = step_no;
   pr->p.im_time = im_time != 0;
+  API_END
+}
+
+extern "C" int so_propagation_set_autotune(so_propagation* pr, int enable) {
+  API_BEGIN
+  NEED(pr);
+  if (pr->p.autotune != (enable != 0)) {
+    prop_reset_compiled(pr->p);
+    pr->p.autotune = enable != 0;
+  }
   API_END
 }
 

@@ -152,7 +152,7 @@ class GroundStateBatch:
     `B` must provide Batch (the HIP backend); `batched=False` steps the same instances one by one."""
 
     def __init__(self, B, count=64, seed0=4000, dt=0.05, max_steps=40000, tol=1e-10, dtype="f64",
-                 sizes=None, batched=True, r0=8.0, sigma=1.0, gemm=None):
+                 sizes=None, batched=True, r0=8.0, sigma=1.0, gemm=None, autotune=False):
         self.B, self.dt, self.max_steps, self.tol = B, dt, max_steps, tol
         self.inst = []
         for i in range(count):
@@ -165,6 +165,8 @@ class GroundStateBatch:
                                            params=prm, r0=r0, sigma=sigma)
             if gemm is not None:
                 p.set_gemm_mode(gemm)
+            if autotune:
+                p.set_autotune(True)
             self.inst.append({"prop": p, "n_r": n_r, "n_theta": n_t, "params": prm, "steps": 0,
                               "energy": float("nan"), "converged": False, "de": float("nan")})
         self.batched = batched

@@ -276,6 +276,26 @@ def test_split_bf16x3_imaginary_time_energy(n_theta):
     assert abs(e[0] - e[1]) < 3e-3 * abs(e[1]), e
 
 
+def test_autotuned_tiles_match(tmp_path, monkeypatch):
+    """Tiles chosen by the autotuner change only the summation order of the norms: the tuned run agrees
+    with the oracle to round-off and with the untuned run to 1e-13; the choices land in SO_TUNE_CACHE."""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("SO_TUNE_CACHE", str(cache))
+    kw = dict(n_r=256, n_theta=32, steps=60, dt=0.002, leak=True)
+    out = []
+    for tune in (True, False):
+        p, _, _ = W.radial_angular(so, dtype="f64", **kw)
+        p.set_autotune(tune)
+        p.propagate()
+        out.append(p.wave_function().array)
+    po, _, _ = W.radial_angular(O, **kw)
+    po.propagate()
+    assert rel_l2(out[0], po.wave_function().array) < 1e-10
+    assert rel_l2(out[0], out[1]) < 1e-13
+    lines = cache.read_text().split()
+    assert len(lines) >= 2 and all(int(v) >= 1 for v in lines[1::2])
+
+
 def test_graphs_replay_matches_eager():
     """hipGraph replay of steady-state steps gives the same state as eager launches."""
     out = []
