@@ -119,6 +119,14 @@ int so_matrix_transformation_create(so_context *ctx, int dimension_no, int64_t n
 /* set_diagonalization_matrix (matrix_transformation.rs:35-51): n x n complex, row-major */
 int so_matrix_transformation_set_matrix(so_op *t, int64_t n, const double *transformation,
                                         const double *inverse_transformation);
+/* StateMatrixTransformation::new (state_matrix_transformation.rs:19-33): lanes along dimension_no, one matrix per
+ * index of dimension_no_dependent (> dimension_no); grid_transformation named like MatrixTransformation's */
+int so_state_matrix_transformation_create(so_context *ctx, int dimension_no, int dimension_no_dependent, int64_t n,
+                                          const char *grid_name, const double *grid_nodes, const double *grid_weights,
+                                          so_op **out);
+/* set_diagonalization_matrices (:36-44): `count` complex n x n matrices each way, [count][n][n] row-major */
+int so_state_matrix_transformation_set_matrices(so_op *t, int64_t count, int64_t n, const double *transformations,
+                                                const double *inverse_transformations);
 /* the held grid_transformation (pub field, fft_transformation.rs:19, matrix_transformation.rs:16) */
 int so_transformation_grid(so_op *t, char *name, int64_t cap, int64_t *n, double *nodes, double *weights);
 /* Transformation::transform / inverse_transform applied directly to a wave function */

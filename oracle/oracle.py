@@ -64,6 +64,11 @@ def lib():
             "orc_kinetic_hamiltonian": (None, [vp, C.c_double, _dp]),
             "orc_legendre_polynomials": (None, [C.c_int64, C.c_double, _dp]),
             "orc_legendre_matrices": (None, [vp, _dp, _dp]),
+            "orc_double_factorial": (C.c_double, [C.c_int64]),
+            "orc_associated_legendre_polynomials": (None, [C.c_int64, C.c_int, C.c_double, _dp]),
+            "orc_legendre_normalization": (C.c_double, [C.c_int64, C.c_int]),
+            "orc_associated_legendre_matrices": (None, [vp, C.c_int, _dp, _dp]),
+            "orc_state_matrix_transformation": (vp, [C.c_int, vp, vp, C.c_int64, _dp, _dp]),
             "orc_rotational_hamiltonian": (None, [vp, vp, C.c_double, C.c_double, _dp]),
             "orc_dispersion": (C.c_double, [C.c_double, C.c_int, C.c_double]),
             "orc_lennard_jones": (C.c_double, [C.c_double] * 3),
@@ -286,6 +291,28 @@ def legendre_matrices(polar):
     return t, ti
 
 
+def double_factorial(n):
+    return lib().orc_double_factorial(n)
+
+
+def associated_legendre_polynomials(j, m, x):
+    out = np.zeros(max(j, abs(m) + 1) + 1)
+    lib().orc_associated_legendre_polynomials(j, m, x, _d(out))
+    return out[: j + 1]
+
+
+def normalization(l, m):
+    return lib().orc_legendre_normalization(l, m)
+
+
+def associated_legendre_matrices(polar, omega):
+    n = polar.nodes_no
+    t = np.zeros((n, n), dtype=np.complex128)
+    ti = np.zeros((n, n), dtype=np.complex128)
+    lib().orc_associated_legendre_matrices(polar._h, int(omega), _d(t.view(np.float64)), _d(ti.view(np.float64)))
+    return t, ti
+
+
 def rotational_hamiltonian(radial, polar, red_mass, rot_const):
     out = np.zeros((radial.nodes_no, polar.nodes_no))
     lib().orc_rotational_hamiltonian(radial._h, polar._h, red_mass, rot_const, _d(out))
@@ -391,6 +418,14 @@ def matrix_transformation(grid, transformed, t, tinv):
     ti = _c128(tinv)
     return Op(lib().orc_matrix_transformation(grid._h, transformed._h, _d(t.view(np.float64)),
                                               _d(ti.view(np.float64))))
+
+
+def state_matrix_transformation(dimension_no_dependent, grid, grid_transformation, transformations,
+                                inverse_transformations):
+    ts = _c128(transformations)
+    tis = _c128(inverse_transformations)
+    return Op(lib().orc_state_matrix_transformation(dimension_no_dependent, grid._h, grid_transformation._h,
+                                                    ts.shape[0], _d(ts.view(np.float64)), _d(tis.view(np.float64))))
 
 
 def non_diag_propagator(dim, ops):

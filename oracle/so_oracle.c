@@ -438,6 +438,78 @@ void orc_legendre_matrices(const orc_grid *polar, double *t, double *tinv) {
     free(pl);
 }
 
+/* special_functions.rs:43-55 — n!! as the crate forms it (n, n-2, ... down to 3 or 2) */
+double orc_double_factorial(int64_t n) {
+    if (n == 0) return 1.0;
+    double v = (double)n;
+    for (int64_t k = n - 2; k >= 2; k -= 2) v *= (double)k;
+    return v;
+}
+
+/* special_functions.rs:57-80 */
+static double negate_m(int64_t l, int m) {
+    double v = (m % 2 == 0) ? 1.0 : -1.0;
+    if (m > 0) {
+        for (int64_t k = l - m; k < l + m; k++) v /= (double)k + 1.0;
+    } else if (m < 0) {
+        for (int64_t k = l + m; k < l - m; k++) v *= (double)k + 1.0;
+    } else {
+        v = 1.0;
+    }
+    return v;
+}
+
+/* special_functions.rs:17-41 — P_l^m(x), l = 0..j (entries below |m| are 0) */
+void orc_associated_legendre_polynomials(int64_t j, int m, double x, double *p) {
+    int64_t mu = m < 0 ? -m : m;
+    if (m == 0) {
+        orc_legendre_polynomials(j, x, p);
+        return;
+    }
+    if (mu + 1 > j) orc_die("associated_legendre_polynomials: index out of bounds (the crate panics)");
+    for (int64_t i = 0; i <= j; i++) p[i] = 0.0;
+    p[mu] = ((m % 2 == 0) ? 1.0 : -1.0) * orc_double_factorial(2 * mu - 1) * pow(1.0 - x * x, (double)mu / 2.0);
+    if (m < 0) {
+        p[mu] *= negate_m(mu, (int)mu);
+        p[mu + 1] = x * p[mu];
+    } else {
+        p[mu + 1] = x * (2.0 * (double)mu + 1.0) * p[mu];
+    }
+    for (int64_t i = mu + 2; i <= j; i++) {
+        int64_t l = i - 1;
+        p[i] = ((double)(2 * l + 1) * x * p[i - 1] - ((double)l + (double)m) * p[i - 2]) / ((double)l - (double)m + 1.0);
+    }
+}
+
+/* special_functions.rs:82-116 — sqrt((l + 1/2) (l - m)! / (l + m)!) */
+double orc_legendre_normalization(int64_t l, int m) {
+    double v = 1.0;
+    if (m > 0) {
+        for (int64_t k = l - m; k < l + m; k++) v /= (double)k + 1.0;
+    } else if (m < 0) {
+        for (int64_t k = l + m; k < l - m; k++) v *= (double)k + 1.0;
+    }
+    return sqrt(v * ((double)l + 0.5));
+}
+
+/* legendre_diagonalization.rs:48-80 — transformation[i][j] = N_l P_l^omega(cos theta_j) w_j for i >= |omega|,
+ * inverse = the unweighted matrix transposed */
+void orc_associated_legendre_matrices(const orc_grid *polar, int omega, double *t, double *tinv) {
+    int64_t n = polar->nodes_no, lmax = n - 1, om = omega < 0 ? -omega : omega;
+    double *pl = (double *)xmalloc(sizeof(double) * (size_t)(lmax + 2));
+    memset(t, 0, sizeof(double) * (size_t)(2 * n * n));
+    memset(tinv, 0, sizeof(double) * (size_t)(2 * n * n));
+    for (int64_t j = 0; j < n; j++) {
+        orc_associated_legendre_polynomials(lmax, omega, cos(polar->nodes[j]), pl);
+        for (int64_t i = om; i < n; i++) {
+            double v = orc_legendre_normalization(i, omega) * pl[i];
+            tinv[2 * (j * n + i)] = v;
+            t[2 * (i * n + j)] = v * polar->weights[j];
+        }
+    }
+    free(pl);
+}
+
 /* rotational_operator.rs:7-26 — (radial x polar) l(l+1)(B + 1/(2 mu r^2)) */
 void orc_rotational_hamiltonian(const orc_grid *radial, const orc_grid *polar, double red_mass,
                                 double rot_const, double *out) {
@@ -641,7 +713,8 @@ enum {
     K_BORDER,
     K_LEAK,
     K_STATE_SAVER,
-    K_WF_SAVER
+    K_WF_SAVER,
+    K_STATE_MATRIX
 };
 
 typedef struct {
@@ -670,8 +743,9 @@ struct orc_op {
     orc_grid *held; /* grid_transformation */
     fft_plan plan;
     orc_c *t, *tinv;
-    /* non-diagonal */
+    /* non-diagonal; state matrix: number of matrices and dependent axis */
     int64_t lanes;
+    int dep;
     /* controls */
     double leak_norm;
     loss_checker chk;
@@ -801,6 +875,26 @@ orc_op *orc_matrix_transformation(const orc_grid *g, const orc_grid *transformed
     o->tinv = (orc_c *)xmalloc(sizeof(orc_c) * nn);
     memcpy(o->t, t, sizeof(orc_c) * nn);
     memcpy(o->tinv, tinv, sizeof(orc_c) * nn);
+    return o;
+}
+
+/* state_matrix_transformation.rs:19-44 — count matrices n x n for each direction */
+orc_op *orc_state_matrix_transformation(int dep, const orc_grid *g, const orc_grid *transformed, int64_t count,
+                                        const double *ts, const double *tinvs) {
+    if (dep <= g->dimension_no) orc_die("Dependent dimension should have higher dimension index");
+    orc_op *o = op_alloc(K_STATE_MATRIX);
+    o->dim = g->dimension_no;
+    o->dep = dep;
+    o->n = g->nodes_no;
+    o->lanes = count;
+    o->held = orc_grid_clone(transformed);
+    size_t sz = (size_t)(count * o->n * o->n);
+    o->t = (orc_c *)xmalloc(sizeof(orc_c) * (sz ? sz : 1));
+    o->tinv = (orc_c *)xmalloc(sizeof(orc_c) * (sz ? sz : 1));
+    if (sz) {
+        memcpy(o->t, ts, sizeof(orc_c) * sz);
+        memcpy(o->tinv, tinvs, sizeof(orc_c) * sz);
+    }
     return o;
 }
 
@@ -1015,11 +1109,47 @@ void orc_op_apply(orc_op *o, orc_wf *wf) {
     if (o->chk.present) chk_after(&o->chk, wf);
 }
 
+/* state_matrix_transformation.rs:52-74 — swap the grid; slice j of the dependent axis has every lane along the
+ * axis replaced by mats[j] . lane (slices past the matrices, or matrices past the slices, are skipped by zip) */
+static void state_matrix_lanes(orc_op *o, orc_wf *wf, const orc_c *mats) {
+    int d = o->dim;
+    orc_grid *tmp = wf->grids[d];
+    wf->grids[d] = o->held;
+    o->held = tmp;
+    wf->possible_norm_change = 1;
+    int64_t outer, inner, n = o->n;
+    if (wf->shape[d] != n) orc_die("StateMatrixTransformation: lane length mismatch");
+    lane_geom(wf, d, &outer, &inner);
+    int64_t dep_stride = 1;
+    for (int a = o->dep + 1; a < wf->ndim; a++) dep_stride *= wf->shape[a];
+    int64_t dep_n = wf->shape[o->dep];
+#pragma omp parallel
+    {
+        orc_c *lane = (orc_c *)xmalloc(sizeof(orc_c) * (size_t)n);
+#pragma omp for collapse(2) schedule(static)
+        for (int64_t a = 0; a < outer; a++)
+            for (int64_t i = 0; i < inner; i++) {
+                int64_t jj = (i / dep_stride) % dep_n;
+                if (jj >= o->lanes) continue;
+                const orc_c *m = mats + jj * n * n;
+                for (int64_t k = 0; k < n; k++) lane[k] = wf->a[(a * n + k) * inner + i];
+                for (int64_t r = 0; r < n; r++) {
+                    orc_c acc = {0.0, 0.0};
+                    for (int64_t k = 0; k < n; k++) acc = c_add(acc, c_mul(m[r * n + k], lane[k]));
+                    wf->a[(a * n + r) * inner + i] = acc;
+                }
+            }
+        free(lane);
+    }
+}
+
 void orc_op_transform(orc_op *o, orc_wf *wf) {
     if (o->kind == K_FFT)
         fft_lanes(o, wf, 0);
     else if (o->kind == K_MATRIX)
         matrix_lanes(o, wf, o->t);
+    else if (o->kind == K_STATE_MATRIX)
+        state_matrix_lanes(o, wf, o->t);
     else
         orc_die("transform on a non-transformation");
 }
@@ -1028,6 +1158,8 @@ void orc_op_inverse_transform(orc_op *o, orc_wf *wf) {
         fft_lanes(o, wf, 1);
     else if (o->kind == K_MATRIX)
         matrix_lanes(o, wf, o->tinv);
+    else if (o->kind == K_STATE_MATRIX)
+        state_matrix_lanes(o, wf, o->tinv);
     else
         orc_die("inverse_transform on a non-transformation");
 }

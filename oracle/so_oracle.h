@@ -75,6 +75,10 @@ double orc_lennard_jones(double r, double d6, double r6);
 double orc_harmonic(double r, double r0, double mass, double omega);
 void orc_dumping_end(double mask_width, double mask_end, const orc_grid *g, double *out);
 void orc_gauss_legendre(int64_t n, double *x, double *w);
+double orc_double_factorial(int64_t n);
+void orc_associated_legendre_polynomials(int64_t j, int m, double x, double *out);
+double orc_legendre_normalization(int64_t l, int m);
+void orc_associated_legendre_matrices(const orc_grid *polar, int omega, double *t, double *tinv);
 
 /* ---- operations ------------------------------------------------------- */
 typedef struct orc_op orc_op;
@@ -85,6 +89,8 @@ const orc_grid *orc_op_held_grid(const orc_op *op);
 orc_op *orc_matrix_transformation(const orc_grid *g, const orc_grid *transformed,
                                   const double *t, const double *tinv);
 orc_op *orc_non_diag_propagator(int dim, int64_t lanes, int64_t n, const double *ops);
+orc_op *orc_state_matrix_transformation(int dep, const orc_grid *g, const orc_grid *transformed, int64_t count,
+                                        const double *ts, const double *tinvs);
 orc_op *orc_border_dumping(const double *mask, const orc_grid *g);
 orc_op *orc_leak_control(void);
 orc_op *orc_state_saver(const char *name, double step, int64_t step_no, const orc_grid *state_grid,

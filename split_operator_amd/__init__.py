@@ -16,10 +16,11 @@ import numpy as np
 
 from . import _lib
 from ._lib import SoError, call, dptr, i64ptr
-from .hamiltonian_factory import (dispersion, dumping_end, exp_propagator, exp_propagator_complex,  # noqa: F401
+from .hamiltonian_factory import (associated_legendre_matrices, associated_legendre_polynomials,  # noqa: F401
+                                  dispersion, double_factorial, dumping_end, exp_propagator, exp_propagator_complex,
                                   gaussian_distribution, harmonic, kinetic_hamiltonian, legendre_matrices,
-                                  legendre_polynomials, lennard_jones, momentum_grid, rotational_hamiltonian,
-                                  select_step, two_dim_into_n_dim_operator)
+                                  legendre_polynomials, lennard_jones, momentum_grid, normalization,
+                                  rotational_hamiltonian, select_step, two_dim_into_n_dim_operator)
 
 APPLY_NONE = 0
 APPLY_FIRST = 1
@@ -339,6 +340,64 @@ def legendre_diagonalization_operator(polar_grid):
     l_grid = Grid.new_linear_countable("angular_momentum", 0.0, float(n - 1), n, polar_grid.dimension_no)
     t, ti = legendre_matrices(polar_grid)
     return matrix_transformation(polar_grid, l_grid, t, ti)
+
+
+def state_matrix_transformation(dimension_no_dependent, grid, grid_transformation, transformations=None,
+                                inverse_transformations=None):
+    """StateMatrixTransformation::new + set_diagonalization_matrices (state_matrix_transformation.rs:19-44):
+    slice j of axis `dimension_no_dependent` has its lanes along `grid`'s axis transformed by matrix j."""
+    h = C.c_void_p()
+    g = grid_transformation
+    call("so_state_matrix_transformation_create", context(), grid.dimension_no, dimension_no_dependent,
+         grid.nodes_no, g.name.encode(), dptr(g.nodes), dptr(g.weights), C.byref(h))
+    o = Op(h)
+    o.dimension_no = grid.dimension_no
+    if transformations is not None:
+        o.set_diagonalization_matrices(transformations, inverse_transformations)
+    return o
+
+
+def _set_state_matrices(self, transformations, inverse_transformations):
+    ts = _c128(transformations)
+    tis = _c128(inverse_transformations)
+    assert ts.shape == tis.shape and ts.ndim == 3 and ts.shape[1] == ts.shape[2]
+    call("so_state_matrix_transformation_set_matrices", self._h, ts.shape[0], ts.shape[1],
+         dptr(ts.view(np.float64)), dptr(tis.view(np.float64)))
+
+
+Op.set_diagonalization_matrices = _set_state_matrices
+
+
+def associated_legendre_diagonalization_operator(polar_grid, omega):
+    """legendre_diagonalization.rs:48-80: MatrixTransformation to the associated Legendre basis of one omega."""
+    n = polar_grid.nodes_no
+    l_grid = Grid.new_linear_countable("angular_momentum", 0.0, float(n - 1), n, polar_grid.dimension_no)
+    t, ti = associated_legendre_matrices(polar_grid, omega)
+    return matrix_transformation(polar_grid, l_grid, t, ti)
+
+
+def associated_legendre_operator(polar_grid, omega_grid):
+    """legendre_diagonalization.rs:83-109: StateMatrixTransformation with one associated Legendre basis per
+    omega node (omega values are the node values, truncated to integers)."""
+    n = polar_grid.nodes_no
+    l_grid = Grid.new_linear_countable("angular_momentum", 0.0, float(n - 1), n, polar_grid.dimension_no)
+    mats = [associated_legendre_matrices(polar_grid, int(om)) for om in omega_grid.nodes]
+    ts = np.stack([m[0] for m in mats])
+    tis = np.stack([m[1] for m in mats])
+    return state_matrix_transformation(omega_grid.dimension_no, polar_grid, l_grid, ts, tis)
+
+
+def non_diag_propagator(dimension_no, operators):
+    """NonDiagPropagator::new + set_operators (non_diagonal_propagator.rs:16-27): operators[k] is applied to
+    lane k along the axis (lanes in C order)."""
+    ops = _c128(operators)
+    assert ops.ndim == 3 and ops.shape[1] == ops.shape[2]
+    h = C.c_void_p()
+    call("so_non_diag_propagator_create", context(), dimension_no, ops.shape[0], ops.shape[1],
+         dptr(ops.view(np.float64)), C.byref(h))
+    o = Op(h)
+    o.dimension_no = dimension_no
+    return o
 
 
 def border_dumping(mask, grid):

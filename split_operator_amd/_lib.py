@@ -51,6 +51,8 @@ SIGNATURES = {
     "so_fft_transformation_create": [_vp, C.c_int, C.c_int64, _dp, C.c_char_p, _vpp],
     "so_matrix_transformation_create": [_vp, C.c_int, C.c_int64, C.c_char_p, _dp, _dp, _vpp],
     "so_matrix_transformation_set_matrix": [_vp, C.c_int64, _dp, _dp],
+    "so_state_matrix_transformation_create": [_vp, C.c_int, C.c_int, C.c_int64, C.c_char_p, _dp, _dp, _vpp],
+    "so_state_matrix_transformation_set_matrices": [_vp, C.c_int64, C.c_int64, _dp, _dp],
     "so_transformation_grid": [_vp, C.c_char_p, C.c_int64, _i64p, _dp, _dp],
     "so_transformation_transform": [_vp, _vp],
     "so_transformation_inverse_transform": [_vp, _vp],

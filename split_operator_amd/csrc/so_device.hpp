@@ -161,6 +161,10 @@ enum FinMode : int32_t { FIN_NONE = 0, FIN_IMMEDIATE = 1, FIN_DEFERRED = 2 };
 extern "C++" {
 int sod_launch_pass(int dtype, const sod::PassDesc* d_desc, const sod::PassDesc& h_desc, hipStream_t s);
 int sod_launch_scalar(const sod::PassDesc* d_desc, hipStream_t s);
+// lanes along an axis times a complex matrix each (mode 0: matrix per lane; mode 1: matrix per index of a
+// dependent axis); matrices in the arithmetic precision, [count][n][n]
+int sod_launch_lanemat(int dtype, void* psi, const void* mats, int64_t n, int64_t outer, int64_t inner, int mode,
+                       int64_t dep_stride, int64_t dep_n, int64_t count, hipStream_t s);
 int sod_pass_lds_bytes(int dtype, const sod::PassDesc& h);
 int sod_launch_batch(int dtype, int nx, int xf_type, const sod::PassDesc* d_descs, const int64_t* d_starts,
                      int ninst, int64_t total_tiles, int ept, size_t lds, hipStream_t s);

@@ -1285,6 +1285,60 @@ __global__ __launch_bounds__(NTHREADS) void elem_batch_kernel(const PassDesc* __
   elem_pass_body<S, T, EPT>(ds + i, b - s0, starts[i + 1] - s0, b == s0);
 }
 
+// A matrix per lane: lane (o, i) along the lane axis (element k at
+// (o n + k) inner + i) becomes M[sel] . lane, sel = o inner + i (mode 0,
+// NonDiagPropagator) or the lane's index along a dependent axis,
+// (i / dep_stride) % dep_n (mode 1, StateMatrixTransformation); lanes with
+// sel >= count are left as they are. A workgroup stages tl lanes in LDS, then
+// each thread forms output elements from the staged lane and the matrix row.
+template <typename S, typename T>
+__global__ __launch_bounds__(NTHREADS) void lanemat_kernel(void* psi_, const void* mats_, int64_t n, int64_t outer,
+                                                           int64_t inner, int mode, int64_t dep_stride, int64_t dep_n,
+                                                           int64_t count, int tl) {
+  using T2 = typename CT<T>::T2;
+  using S2 = typename CT<S>::T2;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T2* x = (T2*)smem;
+  S2* psi = (S2*)psi_;
+  const T2* mats = (const T2*)mats_;
+  const int64_t lanes = outer * inner;
+  const int64_t l0 = (int64_t)blockIdx.x * tl;
+  const int tid = threadIdx.x;
+  const int64_t tot = (int64_t)tl * n;
+  for (int64_t e = tid; e < tot; e += NTHREADS) {
+    const int64_t j = e / n, k = e - j * n;
+    const int64_t l = l0 + j;
+    T2 v;
+    v.x = 0;
+    v.y = 0;
+    if (l < lanes) {
+      const int64_t o = l / inner, i = l - o * inner;
+      v = cvt2<T>(gld(&psi[(o * n + k) * inner + i]));
+    }
+    lst(&x[e], v);
+  }
+  __syncthreads();
+  for (int64_t e = tid; e < tot; e += NTHREADS) {
+    const int64_t j = e / n, r = e - j * n;
+    const int64_t l = l0 + j;
+    if (l >= lanes) continue;
+    const int64_t o = l / inner, i = l - o * inner;
+    const int64_t sel = mode == 0 ? l : (i / dep_stride) % dep_n;
+    if (sel >= count) continue;
+    const T2* row = mats + (sel * n + r) * n;
+    T2 acc;
+    acc.x = 0;
+    acc.y = 0;
+    for (int64_t k = 0; k < n; ++k) {
+      const T2 m = gld(&row[k]);
+      const T2 xv = lld(&x[j * n + k]);
+      acc.x += m.x * xv.x - m.y * xv.y;
+      acc.y += m.x * xv.y + m.y * xv.x;
+    }
+    gst(&psi[(o * n + r) * inner + i], cvt2<S>(acc));
+  }
+}
+
 __global__ __launch_bounds__(NTHREADS) void scalar_kernel(const PassDesc* __restrict__ d) {
   run_inherited(d, threadIdx.x);
   __syncthreads();
@@ -1575,6 +1629,41 @@ int sod_launch_batch(int dtype, int nx, int xf_type, const PassDesc* d_descs, co
     case 0: launch_batch_st<float, float>(nx, xf_type, d_descs, d_starts, ninst, ept, grid, lds, s); break;
     case 1: launch_batch_st<double, double>(nx, xf_type, d_descs, d_starts, ninst, ept, grid, lds, s); break;
     case 2: launch_batch_st<float, double>(nx, xf_type, d_descs, d_starts, ninst, ept, grid, lds, s); break;
+    default: return (int)hipErrorInvalidValue;
+  }
+  return (int)hipGetLastError();
+}
+
+int sod_launch_lanemat(int dtype, void* psi, const void* mats, int64_t n, int64_t outer, int64_t inner, int mode,
+                       int64_t dep_stride, int64_t dep_n, int64_t count, hipStream_t s) {
+  const int64_t lanes = outer * inner;
+  int tl = (int)std::max<int64_t>(1, std::min<int64_t>(64, 2048 / std::max<int64_t>(n, 1)));
+  const size_t lds = (size_t)tl * n * (dtype == 0 ? 8 : 16);
+  if (lds > SOD_MAX_DYN_LDS || lanes < 1) return (int)hipErrorInvalidValue;
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void*)lanemat_kernel<float, float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)SOD_MAX_DYN_LDS);
+    (void)hipFuncSetAttribute((const void*)lanemat_kernel<double, double>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)SOD_MAX_DYN_LDS);
+    (void)hipFuncSetAttribute((const void*)lanemat_kernel<float, double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)SOD_MAX_DYN_LDS);
+    attr_set = true;
+  }
+  const dim3 grid((unsigned)((lanes + tl - 1) / tl));
+  switch (dtype) {
+    case 0:
+      lanemat_kernel<float, float><<<grid, NTHREADS, lds, s>>>(psi, mats, n, outer, inner, mode, dep_stride, dep_n,
+                                                               count, tl);
+      break;
+    case 1:
+      lanemat_kernel<double, double><<<grid, NTHREADS, lds, s>>>(psi, mats, n, outer, inner, mode, dep_stride, dep_n,
+                                                                 count, tl);
+      break;
+    case 2:
+      lanemat_kernel<float, double><<<grid, NTHREADS, lds, s>>>(psi, mats, n, outer, inner, mode, dep_stride, dep_n,
+                                                                count, tl);
+      break;
     default: return (int)hipErrorInvalidValue;
   }
   return (int)hipGetLastError();

@@ -115,7 +115,18 @@ static bool radix_plan(int64_t n, std::vector<int>& rad) {
 }
 
 // ============================================================== ops
-enum OpKind { OK_ONE_DIM = 1, OK_N_DIM, OK_FFT, OK_MATRIX, OK_NON_DIAG, OK_BORDER, OK_LEAK, OK_STATE_SAVER, OK_WF_SAVER };
+enum OpKind {
+  OK_ONE_DIM = 1,
+  OK_N_DIM,
+  OK_FFT,
+  OK_MATRIX,
+  OK_NON_DIAG,
+  OK_BORDER,
+  OK_LEAK,
+  OK_STATE_SAVER,
+  OK_WF_SAVER,
+  OK_STATE_MATRIX
+};
 
 struct LossCheckerH {
   bool present = false;
@@ -201,7 +212,10 @@ struct so_op {
   GridH held;  // grid_transformation
   std::vector<int> radices;
   DevTable twiddles;
-  std::vector<double> mat_t, mat_tinv;  // complex n x n
+  std::vector<double> mat_t, mat_tinv;  // complex n x n (NonDiag / StateMatrix: count of them)
+  int dep = -1;                           // StateMatrix: dependent axis
+  int64_t count = 0;                      // NonDiag / StateMatrix: number of matrices
+  DevTable table2;                        // StateMatrix: inverse matrices on the device
   void* gemm_b[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [which][dtype]
   void* gemm_bh[2] = {nullptr, nullptr};  // [which]: split-BF16 high parts of the FP32 operand
   void* gemm_bl[2] = {nullptr, nullptr};  // [which]: low parts
@@ -429,6 +443,8 @@ struct Launch {
   CompiledPass* pass = nullptr;
   SaveAct save;
   bool is_save = false;
+  so_op* lanemat = nullptr;  // a matrix-per-lane op launched on its own (NonDiag, StateMatrix)
+  int which = 0;             // StateMatrix: 0 transform, 1 inverse
 };
 
 struct Propagation {
@@ -539,6 +555,13 @@ struct Propagation {
     open.nx = 1;
     open.stage = 1;
   }
+  void emit_lanemat(so_op* op, int which) {
+    close_open();
+    Launch L;
+    L.lanemat = op;
+    L.which = which;
+    emitted.push_back(L);
+  }
   void emit_save(so_op* sv, int64_t col) {
     close_open();
     Launch L;
@@ -625,9 +648,17 @@ struct Propagation {
   }
   // Propagator::apply (one_dim_propagator.rs:55-65, n_dim_propagator.rs:44-54)
   void sim_apply(so_op* op) {
-    if (op->kind == OK_NON_DIAG) fail(SO_ERR_UNSUPPORTED, "NonDiagPropagator is not on the device path yet");
     if (op->chk.present) sim_check_before(op->chk);
-    sim_diag(op);
+    if (op->kind == OK_NON_DIAG) {
+      // non_diagonal_propagator.rs:29-37: lane k of the axis <- operators[k] . lane k
+      if (op->dim >= wf->ndim || wf->shape[op->dim] != op->n)
+        fail(SO_ERR_SHAPE, "NonDiagPropagator of %lld x %lld matrices does not match axis %d", (long long)op->n,
+             (long long)op->n, op->dim);
+      wf->possible_norm_change = true;
+      emit_lanemat(op, 0);
+    } else {
+      sim_diag(op);
+    }
     if (op->chk.present) sim_check_after(op->chk);
   }
   // FFTTransformation / MatrixTransformation transform (fft_transformation.rs:53-88,
@@ -637,6 +668,16 @@ struct Propagation {
     if (wf->shape[op->dim] != op->n)
       fail(SO_ERR_SHAPE, "transformation of length %lld on axis %d of length %lld", (long long)op->n, op->dim,
            (long long)wf->shape[op->dim]);
+    if (op->kind == OK_STATE_MATRIX) {
+      // state_matrix_transformation.rs:52-74: swap the grid, then every slice j of the dependent axis has its
+      // lanes along the axis multiplied by transformations[j] (or the inverse)
+      if (op->dep >= wf->ndim) fail(SO_ERR_SHAPE, "dependent axis %d out of range", op->dep);
+      if (op->mat_t.empty() && op->count > 0) fail(SO_ERR_STATE, "StateMatrixTransformation matrices not set");
+      std::swap(wf->grids[op->dim], op->held);
+      wf->possible_norm_change = true;
+      emit_lanemat(op, inverse ? 1 : 0);
+      return;
+    }
     std::swap(wf->grids[op->dim], op->held);
     wf->possible_norm_change = true;
     XSpec x;
@@ -1258,6 +1299,29 @@ void Propagation::launch(const Launch& L) {
     run_saver(L.save);
     return;
   }
+  if (L.lanemat) {
+    so_op* op = L.lanemat;
+    const int ax = op->dim;
+    int64_t outer = 1, inner = 1;
+    for (int a = 0; a < ax; ++a) outer *= wf->shape[a];
+    for (int a = ax + 1; a < wf->ndim; ++a) inner *= wf->shape[a];
+    const int ad = arith_dt(dtype());
+    int mode = 0;
+    int64_t dep_stride = 1, dep_n = 1;
+    const void* mats;
+    if (op->kind == OK_STATE_MATRIX) {
+      mode = 1;
+      for (int a = op->dep + 1; a < wf->ndim; ++a) dep_stride *= wf->shape[a];
+      dep_n = wf->shape[op->dep];
+      mats = L.which ? op->table2.get(ad) : op->table.get(ad);
+    } else {
+      mats = op->table.get(ad);
+    }
+    if (op->count > 0)
+      HIPCHK((hipError_t)sod_launch_lanemat(dtype(), wf->data, mats, op->n, outer, inner, mode, dep_stride, dep_n,
+                                            op->count, ctx->stream));
+    return;
+  }
   launch_pass(L.pass);
 }
 
@@ -1306,6 +1370,12 @@ static std::string launches_key(const std::vector<Launch>& v) {
   for (const auto& L : v) {
     if (L.is_save) {
       k.push_back('S');
+      continue;
+    }
+    if (L.lanemat) {
+      k.push_back('M');
+      key_add(k, &L.lanemat, sizeof(void*));
+      k.push_back((char)L.which);
       continue;
     }
     key_add(k, &L.pass, sizeof(void*));
@@ -1760,10 +1830,14 @@ extern "C" int so_non_diag_propagator_create(so_context* ctx, int dim, int64_t l
   NEED(ctx);
   NEED(ops);
   NEED(out);
+  if (n < 1 || lanes < 0) fail(SO_ERR_INVALID, "bad NonDiagPropagator sizes");
   so_op* o = op_new(ctx, OK_NON_DIAG);
   o->dim = dim;
   o->n = n;
+  o->count = lanes;
   o->mat_t.assign(ops, ops + 2 * lanes * n * n);
+  o->table.host = o->mat_t;
+  o->table.ctx = ctx;
   *out = o;
   API_END
 }
@@ -1958,10 +2032,51 @@ extern "C" int so_matrix_transformation_set_matrix(so_op* t, int64_t n, const do
   API_END
 }
 
+extern "C" int so_state_matrix_transformation_create(so_context* ctx, int dim, int dep, int64_t n, const char* gname,
+                                                     const double* gnodes, const double* gweights, so_op** out) {
+  API_BEGIN
+  NEED(ctx);
+  NEED(gname);
+  NEED(gnodes);
+  NEED(gweights);
+  NEED(out);
+  if (n < 1) fail(SO_ERR_INVALID, "bad length");
+  if (dep <= dim) fail(SO_ERR_INVALID, "Dependent dimension should have higher dimension index");
+  so_op* o = op_new(ctx, OK_STATE_MATRIX);
+  o->dim = dim;
+  o->dep = dep;
+  o->n = n;
+  o->held.name = gname;
+  o->held.dimension_no = dim;
+  o->held.nodes.assign(gnodes, gnodes + n);
+  o->held.weights.assign(gweights, gweights + n);
+  *out = o;
+  API_END
+}
+
+extern "C" int so_state_matrix_transformation_set_matrices(so_op* t, int64_t count, int64_t n, const double* ts,
+                                                           const double* tinvs) {
+  API_BEGIN
+  NEED(t);
+  if (t->kind != OK_STATE_MATRIX) fail(SO_ERR_INVALID, "not a StateMatrixTransformation");
+  if (n != t->n) fail(SO_ERR_SHAPE, "matrices of size %lld for lanes of %lld", (long long)n, (long long)t->n);
+  if (count < 0 || (count > 0 && (!ts || !tinvs))) fail(SO_ERR_INVALID, "bad matrices");
+  t->count = count;
+  t->mat_t.assign(ts, ts + 2 * count * n * n);
+  t->mat_tinv.assign(tinvs, tinvs + 2 * count * n * n);
+  t->table.invalidate();
+  t->table2.invalidate();
+  t->table.host = t->mat_t;
+  t->table.ctx = t->ctx;
+  t->table2.host = t->mat_tinv;
+  t->table2.ctx = t->ctx;
+  API_END
+}
+
 extern "C" int so_transformation_grid(so_op* t, char* name, int64_t cap, int64_t* n, double* nodes, double* weights) {
   API_BEGIN
   NEED(t);
-  if (t->kind != OK_FFT && t->kind != OK_MATRIX) fail(SO_ERR_INVALID, "not a transformation");
+  if (t->kind != OK_FFT && t->kind != OK_MATRIX && t->kind != OK_STATE_MATRIX) fail(SO_ERR_INVALID, "not a transformation");
   const GridH& g = t->held;
   if (name && cap > 0) snprintf(name, (size_t)cap, "%s", g.name.c_str());
   if (n) *n = (int64_t)g.nodes.size();
@@ -2157,6 +2272,7 @@ static void op_free(so_op* o) {
   if (!o) return;
   so_context* c = o->ctx;
   o->table.invalidate();
+  o->table2.invalidate();
   o->twiddles.invalidate();
   for (int w = 0; w < 2; ++w) {
     for (int d = 0; d < 2; ++d) c->release(o->gemm_b[w][d]);
@@ -2210,7 +2326,7 @@ extern "C" int so_operation_stack_add_propagator(so_operation_stack* s, so_op* p
 extern "C" int so_operation_stack_add_transformation(so_operation_stack* s, so_op* t, int order) {
   API_BEGIN
   NEED(t);
-  if (t->kind != OK_FFT && t->kind != OK_MATRIX) fail(SO_ERR_INVALID, "not a transformation");
+  if (t->kind != OK_FFT && t->kind != OK_MATRIX && t->kind != OK_STATE_MATRIX) fail(SO_ERR_INVALID, "not a transformation");
   if (order != SO_ORDER_NORMAL && order != SO_ORDER_INVERSE_FIRST) fail(SO_ERR_INVALID, "bad order");
   stack_push(s, t, 1, order);
   API_END
@@ -2739,10 +2855,11 @@ static std::vector<BatchItem> batch_plan(so_batch* b, std::vector<std::vector<La
     const Launch& L0 = ls[0][k];
     for (auto& v : ls) {
       const Launch& L = v[k];
-      if (L.is_save != L0.is_save)
+      if (L.is_save != L0.is_save || (L.lanemat != nullptr) != (L0.lanemat != nullptr))
         same = false;
-      else if (!L.is_save && (L.pass->scalar_only != L0.pass->scalar_only || L.pass->h.nx != L0.pass->h.nx ||
-                              (L.pass->h.nx && L.pass->h.xf[0].type != L0.pass->h.xf[0].type)))
+      else if (!L.is_save && !L.lanemat &&
+               (L.pass->scalar_only != L0.pass->scalar_only || L.pass->h.nx != L0.pass->h.nx ||
+                (L.pass->h.nx && L.pass->h.xf[0].type != L0.pass->h.xf[0].type)))
         same = false;
     }
   }
@@ -2752,7 +2869,7 @@ static std::vector<BatchItem> batch_plan(so_batch* b, std::vector<std::vector<La
     return plan;
   }
   for (size_t k = 0; k < K; ++k) {
-    if (ls[0][k].is_save || ls[0][k].pass->scalar_only) {
+    if (ls[0][k].is_save || ls[0][k].lanemat || ls[0][k].pass->scalar_only) {
       for (size_t i = 0; i < M; ++i) plan.push_back(BatchItem{nullptr, (int)i, ls[i][k]});
       continue;
     }
@@ -2784,6 +2901,8 @@ static std::string plan_key(const std::vector<BatchItem>& plan) {
     key_add(k, &it.member, sizeof(int));
     if (it.L.is_save) k.push_back('S');
     key_add(k, &it.L.pass, sizeof(void*));
+    key_add(k, &it.L.lanemat, sizeof(void*));
+    k.push_back((char)it.L.which);
   }
   return k;
 }

@@ -76,6 +76,77 @@ def legendre_matrices(polar_grid):
     return np.ascontiguousarray(t), tinv
 
 
+def double_factorial(n):
+    """special_functions.rs:43-55: n (n-2) (n-4) ... down to 3 or 2; 1 for n = 0."""
+    if n >= 100:
+        raise ValueError("double_factorial needs n < 100 (special_functions.rs:44)")
+    v = float(n) if n > 0 else 1.0
+    for k in range(n - 2, 1, -2):
+        v *= k
+    return v
+
+
+def _negate_m(l, m):
+    """special_functions.rs:57-80."""
+    v = -1.0 if m % 2 else 1.0
+    if m > 0:
+        for k in range(l - m, l + m):
+            v /= k + 1.0
+    elif m < 0:
+        for k in range(l + m, l - m):
+            v *= k + 1.0
+    else:
+        v = 1.0
+    return v
+
+
+def associated_legendre_polynomials(j, m, x):
+    """special_functions.rs:17-41: P_l^m(x) for l = 0..j (Condon-Shortley phase), upward in l; 0 below |m|."""
+    mu = abs(m)
+    if m == 0:
+        return legendre_polynomials(j, x)
+    if mu + 1 > j:
+        raise IndexError("associated_legendre_polynomials needs j > |m| (index out of bounds in the crate)")
+    p = np.zeros(j + 1)
+    p[mu] = (-1.0 if m % 2 else 1.0) * double_factorial(2 * mu - 1) * (1.0 - x * x) ** (mu / 2.0)
+    if m < 0:
+        p[mu] *= _negate_m(mu, mu)
+        p[mu + 1] = x * p[mu]
+    else:
+        p[mu + 1] = x * (2.0 * mu + 1.0) * p[mu]
+    for i in range(mu + 2, j + 1):
+        l = i - 1
+        p[i] = ((2 * l + 1) * x * p[i - 1] - (l + m) * p[i - 2]) / (l - m + 1.0)
+    return p
+
+
+def normalization(l, m):
+    """special_functions.rs:82-116: sqrt((l + 1/2) (l - m)! / (l + m)!)."""
+    v = 1.0
+    if m > 0:
+        for k in range(l - m, l + m):
+            v /= k + 1.0
+    elif m < 0:
+        for k in range(l + m, l - m):
+            v *= k + 1.0
+    return math.sqrt(v * (l + 0.5))
+
+
+def associated_legendre_matrices(polar_grid, omega):
+    """legendre_diagonalization.rs:48-80: T[i,j] = N_l P_l^omega(cos th_j) w_j for l_i >= |omega| (0 above),
+    Tinv = the unweighted matrix transposed."""
+    n = polar_grid.nodes_no
+    om = abs(int(omega))
+    tr = np.zeros((n, n), dtype=np.complex128)
+    for j in range(n):
+        pl = associated_legendre_polynomials(n - 1, int(omega), math.cos(polar_grid.nodes[j]))
+        for i in range(om, n):
+            tr[i, j] = normalization(i, int(omega)) * pl[i]
+    tinv = np.ascontiguousarray(tr.T)
+    t = tr * polar_grid.weights[None, :]
+    return np.ascontiguousarray(t), tinv
+
+
 def rotational_hamiltonian(radial_grid, polar_grid, red_mass, rot_const):
     """rotational_operator.rs:7-26: H[r, l] = l(l+1)(B + 1/(2 mu r^2))."""
     l = np.arange(polar_grid.nodes_no, dtype=np.float64)
