@@ -224,12 +224,12 @@ def learned_warm_start(so, args):
     e_ref = [it["hist"][-1][2] for it in ref]
     del ref
     log("cold run")
-    cold, t_cold = ws.relax(peval, 36864, e_ref=e_ref)
+    cold, t_cold = ws.relax(peval, 49152, e_ref=e_ref)
     t0 = time.perf_counter()
     psi0 = ws.predict(peval)
     t_inf = time.perf_counter() - t0
     log("warm run")
-    warm, t_warm = ws.relax(peval, 36864, e_ref=e_ref, psi0s=psi0)
+    warm, t_warm = ws.relax(peval, 49152, e_ref=e_ref, psi0s=psi0)
     met = lambda runs: [it["met"] for it in runs]  # noqa: E731
     ok_c, ok_w = all(m is not None for m in met(cold)), all(m is not None for m in met(warm))
     err_w = max((it["hist"][-1][2] - e) / abs(e) for it, e in zip(warm, e_ref))
