@@ -211,27 +211,43 @@ __device__ __forceinline__ void dft(T2* v, T sg) {
 // across the 64 LDS banks.
 __device__ __forceinline__ int fpad(int i) { return i + (i >> 5); }
 
+// Orders the LDS accesses of one wave: its earlier LDS writes are complete and
+// visible to all of its lanes before its later LDS accesses.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One Stockham radix-R pass over `tl` lanes of length n held in LDS (row pitch
 // ld, padded indices). Butterfly j of a lane reads x[j + r n/R], twiddles by
 // W_{Ls R}^{r k} (k = j mod Ls) and writes y[(j/Ls) Ls R + k + r Ls]. The
 // twiddles of the pass are laid out as tw[(r-1) Ls + k], so the lanes of a
 // wave read consecutive entries. All reads land in registers before the
-// barrier, so the pass is done in place.
-template <int R, int EPT, typename T, typename T2>
+// barrier, so the pass is done in place. `wl` (wave-local, uniform): each wave
+// owns tl/4 whole lanes, so the passes synchronise the wave only; the last
+// pass ends with a workgroup barrier.
+template <int R, int EPT, bool P2, typename T, typename T2>
 __device__ __forceinline__ void radix_pass(T2* lds, int tl, int n, int ld, int Ls, const T2* tw, T scale,
-                                           T scale_lo, T sg, bool last, int tid) {
+                                           T scale_lo, T sg, bool last, int tid, bool wl) {
   constexpr int MAXB = EPT / R > 0 ? EPT / R : 1;
   const int nb = n / R;
-  const int total = tl * nb;
-  // power-of-two strides (every pass before the radix-3 ones) use shifts
-  const bool pow2 = ((Ls & (Ls - 1)) == 0) && ((nb & (nb - 1)) == 0);
+  const int lw = wl ? tl / (NTHREADS / 64) : tl;  // lanes of this wave (wl) or of the workgroup
+  const int l0 = wl ? (tid >> 6) * lw : 0;
+  const int total = lw * nb;
+  const int bstride = wl ? 64 : NTHREADS;
+  if (wl) tid &= 63;
+  // power-of-two strides (every pass before the radix-3 ones, and every pass
+  // when P2 says the lane length is a power of two) use shifts
+  const bool pow2 = P2 || (((Ls & (Ls - 1)) == 0) && ((nb & (nb - 1)) == 0));
   const int ls_sh = __builtin_ctz((unsigned)Ls);
   const int nb_sh = __builtin_ctz((unsigned)nb);
   T2 v[MAXB][R];
   int dsti[MAXB], drow[MAXB];
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
-    const int b = tid + q * NTHREADS;
+    const int b = tid + q * bstride;
     dsti[q] = 0;
     drow[q] = 0;
     if (b < total) {
@@ -247,6 +263,7 @@ __device__ __forceinline__ void radix_pass(T2* lds, int tl, int n, int ld, int L
         jd = j / Ls;
         k = j - jd * Ls;
       }
+      l += l0;
       const T2* src = lds + l * ld;
 #pragma unroll
       for (int r = 0; r < R; ++r) v[q][r] = lld(&src[fpad(j + r * nb)]);
@@ -263,10 +280,13 @@ __device__ __forceinline__ void radix_pass(T2* lds, int tl, int n, int ld, int L
       dsti[q] = jd * Ls * R + k;
     }
   }
-  __syncthreads();
+  if (wl)
+    wave_lds_sync();
+  else
+    __syncthreads();
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
-    const int b = tid + q * NTHREADS;
+    const int b = tid + q * bstride;
     if (b < total) {
       T2* dst = lds + drow[q];
 #pragma unroll
@@ -282,12 +302,93 @@ __device__ __forceinline__ void radix_pass(T2* lds, int tl, int n, int ld, int L
       }
     }
   }
-  __syncthreads();
+  if (wl && !last)
+    wave_lds_sync();
+  else
+    __syncthreads();
 }
 
-template <int EPT, typename T, typename T2>
+// radix_pass for a power-of-two lane length (so n, R, nb = n/R and Ls are all
+// powers of two). Then the padded index is linear in r: fpad(j + r nb) =
+// fpad(j) + r nb + (r nb >> 5), because j < nb and the low five bits of j + r nb
+// never carry (j < nb when nb <= 32, r nb a multiple of 32 otherwise); the same
+// holds for the writes y[(j/Ls) Ls R + k + r Ls]. Each butterfly computes one
+// read and one write address and adds uniform offsets, instead of padding every
+// index. Threads past the last butterfly repeat it (b clamped) and skip the
+// writes, so the values stay unconditionally defined in registers.
+template <int R, int EPT, typename T, typename T2>
+__device__ __forceinline__ void radix_pass_p2(T2* lds, int tl, int n, int ld, int Ls, const T2* tw, T scale,
+                                              T scale_lo, T sg, bool last, int tid, bool wl) {
+  constexpr int MAXB = EPT / R > 0 ? EPT / R : 1;
+  const int nb = n / R;
+  const int lw = wl ? tl / (NTHREADS / 64) : tl;
+  const int l0 = wl ? (tid >> 6) * lw : 0;
+  const int total = lw * nb;
+  const int bstride = wl ? 64 : NTHREADS;
+  if (wl) tid &= 63;
+  const int ls_sh = __builtin_ctz((unsigned)Ls);
+  const int nb_sh = __builtin_ctz((unsigned)nb);
+  T2 v[MAXB][R];
+  int dpos[MAXB];
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int b = min(tid + q * bstride, total - 1);
+    const int l = (b >> nb_sh) + l0;
+    const int j = b & (nb - 1);
+    const int k = j & (Ls - 1);
+    const int jd = j >> ls_sh;
+    const T2* src = lds + l * ld + fpad(j);
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[q][r] = lld(&src[r * nb + ((r * nb) >> 5)]);
+    if (Ls > 1) {
+      const T2* twk = tw + k;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        T2 w = lld(&twk[(r - 1) * Ls]);
+        w.y *= sg;
+        v[q][r] = cmul(v[q][r], w);
+      }
+    }
+    dft<R>(v[q], sg);
+    dpos[q] = l * ld + fpad((jd << ls_sh) * R + k);
+  }
+  if (wl)
+    wave_lds_sync();
+  else
+    __syncthreads();
+  const bool exact = scale_lo == (T)0;
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int b = tid + q * bstride;
+    if (b < total) {
+      T2* dst = lds + dpos[q];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        T2 z = v[q][r];
+        if (last) {
+          // z * (scale + scale_lo): the rounding of 1/sqrt(n) to T does not
+          // accumulate into a norm drift over many transforms
+          if (exact) {
+            z.x *= scale;
+            z.y *= scale;
+          } else {
+            z.x = fma(z.x, scale, z.x * scale_lo);
+            z.y = fma(z.y, scale, z.y * scale_lo);
+          }
+        }
+        lst(&dst[r * Ls + ((r * Ls) >> 5)], z);
+      }
+    }
+  }
+  if (wl && !last)
+    wave_lds_sync();
+  else
+    __syncthreads();
+}
+
+template <int EPT, typename T, bool P2 = false, typename T2>
 __device__ __forceinline__ void fft_lds(T2* lds, int tl, int n, int ld, const T2* tw, double scale_d, bool inverse,
-                                        int nrad, const int32_t* rad, int tid) {
+                                        int nrad, const int32_t* rad, int tid, bool wl = false) {
   const T scale = (T)scale_d;
   const T scale_lo = (T)(scale_d - (double)scale);
   const T sg = inverse ? (T)-1 : (T)1;
@@ -297,14 +398,21 @@ __device__ __forceinline__ void fft_lds(T2* lds, int tl, int n, int ld, const T2
     const int R = rad[p];
     const bool last = (p == nrad - 1);
     const T2* twp = tw + off;
-    if (R == 8)
-      radix_pass<8, EPT>(lds, tl, n, ld, Ls, twp, scale, scale_lo, sg, last, tid);
+    if (P2) {
+      if (R == 8)
+        radix_pass_p2<8, EPT>(lds, tl, n, ld, Ls, twp, scale, scale_lo, sg, last, tid, wl);
+      else if (R == 4)
+        radix_pass_p2<4, EPT>(lds, tl, n, ld, Ls, twp, scale, scale_lo, sg, last, tid, wl);
+      else
+        radix_pass_p2<2, EPT>(lds, tl, n, ld, Ls, twp, scale, scale_lo, sg, last, tid, wl);
+    } else if (R == 8)
+      radix_pass<8, EPT, P2>(lds, tl, n, ld, Ls, twp, scale, scale_lo, sg, last, tid, wl);
     else if (R == 4)
-      radix_pass<4, EPT>(lds, tl, n, ld, Ls, twp, scale, scale_lo, sg, last, tid);
-    else if (R == 3)
-      radix_pass<3, EPT>(lds, tl, n, ld, Ls, twp, scale, scale_lo, sg, last, tid);
+      radix_pass<4, EPT, P2>(lds, tl, n, ld, Ls, twp, scale, scale_lo, sg, last, tid, wl);
+    else if (!P2 && R == 3)
+      radix_pass<3, EPT, P2>(lds, tl, n, ld, Ls, twp, scale, scale_lo, sg, last, tid, wl);
     else
-      radix_pass<2, EPT>(lds, tl, n, ld, Ls, twp, scale, scale_lo, sg, last, tid);
+      radix_pass<2, EPT, P2>(lds, tl, n, ld, Ls, twp, scale, scale_lo, sg, last, tid, wl);
     off += (R - 1) * Ls;
     Ls *= R;
   }
@@ -1077,6 +1185,41 @@ __global__ __launch_bounds__(NTHREADS) void elem_pass_kernel(const PassDesc* __r
 }
 
 // LDS position of element k of a lane: padded in FFT passes, plain in matrix passes
+// Copies the small per-pass arrays (twiddles, 1-D tables, norm weights) into
+// LDS. All loads of a chunk are issued before any store, so the latency is paid
+// once per chunk.
+__device__ __forceinline__ void stage_lds_arrays(const PassDesc* __restrict__ d, unsigned char* smem, int tid) {
+  const int na = d->n_lds_arr;
+  int maxw = 0;
+  for (int a = 0; a < na; ++a) maxw = max(maxw, d->lds_arr_bytes[a] >> 3);
+  constexpr int CW = 4;
+  for (int c0 = 0; c0 < maxw; c0 += CW * NTHREADS) {
+    uint2 tmp[MAX_LDS_ARR][CW];
+#pragma unroll
+    for (int a = 0; a < MAX_LDS_ARR; ++a)
+      if (a < na) {
+        const uint2* src = (const uint2*)d->lds_arr_src[a];
+        const int nw = d->lds_arr_bytes[a] >> 3;
+#pragma unroll
+        for (int q = 0; q < CW; ++q) {
+          const int i = c0 + q * NTHREADS + tid;
+          if (i < nw) tmp[a][q] = gld(&src[i]);
+        }
+      }
+#pragma unroll
+    for (int a = 0; a < MAX_LDS_ARR; ++a)
+      if (a < na) {
+        uint2* dst = (uint2*)(smem + d->lds_arr_off[a]);
+        const int nw = d->lds_arr_bytes[a] >> 3;
+#pragma unroll
+        for (int q = 0; q < CW; ++q) {
+          const int i = c0 + q * NTHREADS + tid;
+          if (i < nw) lst(&dst[i], tmp[a][q]);
+        }
+      }
+  }
+}
+
 template <int KIND>
 __device__ __forceinline__ int lidx(int k) {
   if constexpr (KIND == XF_FFT)
@@ -1160,39 +1303,8 @@ __device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, 
       }
   }
   // stage small arrays (twiddles, 1-D tables, weights) into LDS while the psi
-  // loads above are in flight: all loads of a chunk are issued before any
-  // store, so the latency is paid once
-  {
-    const int na = d->n_lds_arr;
-    int maxw = 0;
-    for (int a = 0; a < na; ++a) maxw = max(maxw, d->lds_arr_bytes[a] >> 3);
-    constexpr int CW = 4;
-    for (int c0 = 0; c0 < maxw; c0 += CW * NTHREADS) {
-      uint2 tmp[MAX_LDS_ARR][CW];
-#pragma unroll
-      for (int a = 0; a < MAX_LDS_ARR; ++a)
-        if (a < na) {
-          const uint2* src = (const uint2*)d->lds_arr_src[a];
-          const int nw = d->lds_arr_bytes[a] >> 3;
-#pragma unroll
-          for (int q = 0; q < CW; ++q) {
-            const int i = c0 + q * NTHREADS + tid;
-            if (i < nw) tmp[a][q] = gld(&src[i]);
-          }
-        }
-#pragma unroll
-      for (int a = 0; a < MAX_LDS_ARR; ++a)
-        if (a < na) {
-          uint2* dst = (uint2*)(smem + d->lds_arr_off[a]);
-          const int nw = d->lds_arr_bytes[a] >> 3;
-#pragma unroll
-          for (int q = 0; q < CW; ++q) {
-            const int i = c0 + q * NTHREADS + tid;
-            if (i < nw) lst(&dst[i], tmp[a][q]);
-          }
-        }
-    }
-  }
+  // loads above are in flight
+  stage_lds_arrays(d, smem, tid);
   __syncthreads();
   if (inherit) run_inherited(d, tid);
   Acc acc;
@@ -1248,6 +1360,144 @@ __global__ __launch_bounds__(NTHREADS) void xform_pass_kernel(const PassDesc* __
     return;
   }
   xform_pass_body<S, T, KIND, EPT>(d, b, ntiles, b == 0 && !inherit_block);
+}
+
+// FFT pass for power-of-two lane lengths with little register state, so that
+// four or more workgroups stay resident per CU and one tile's memory phase
+// overlaps another's transforms. Between stages the tile lives in LDS only;
+// the stages run over chunks of FCH elements per thread; the element ->
+// (lane, index) map is recomputed with shifts (tl and n are powers of two)
+// instead of being held in registers, and the tile's global addresses follow
+// from one origin: lanes are consecutive outer rows (inner == 1, element
+// (l, k) at l n + k) or consecutive inner indices (element (l, k) at
+// l + k inner). PFN = number of full-grid tables prefetched into registers.
+// Same stages, transforms and reductions as xform_pass_body.
+constexpr int FCH = 4;
+
+template <typename S, typename T, int EPT, int PFN>
+__device__ __forceinline__ void fft_lean_body(const PassDesc* __restrict__ d, int64_t tile, int64_t ntiles,
+                                              bool inherit) {
+  using T2 = typename CT<T>::T2;
+  using S2 = typename CT<S>::T2;
+  constexpr int PF1 = PFN > 0 ? PFN : 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ LaneTab lt;
+  const int tid = threadIdx.x;
+  const int tl = d->tl;
+  const int n = (int)d->n;
+  const int ld = d->ld;
+  const int64_t inner = d->inner;
+  const int tot = tl * n;
+  const bool strided = inner > 1;
+  const int nsh = __builtin_ctz((unsigned)n);
+  const int tsh = __builtin_ctz((unsigned)tl);
+  int64_t base0;
+  int nvalid;
+  if (strided) {
+    const int64_t o = tile / d->tiles_per_outer;
+    const int64_t i0 = (tile - o * d->tiles_per_outer) * tl;
+    base0 = o * (int64_t)n * inner + i0;
+    nvalid = (int)min((int64_t)tl, inner - i0);
+  } else {
+    base0 = tile * tl * (int64_t)n;
+    nvalid = (int)min((int64_t)tl, d->outer - tile * tl);
+  }
+  auto lane_of = [&](int f) { return strided ? (f & (tl - 1)) : (f >> nsh); };
+  auto idx_of = [&](int f) { return strided ? (f >> tsh) : (f & (n - 1)); };
+  auto goff = [&](int f) { return strided ? (int64_t)(f & (tl - 1)) + (int64_t)(f >> tsh) * inner : (int64_t)f; };
+  auto live = [&](int f) { return f < tot && lane_of(f) < nvalid; };
+  T2* cur = (T2*)smem;
+  const T2* twl = (const T2*)(smem + d->lds_arr_off[0]);
+  // every global read of the tile is issued first
+  const S2* psi_in = (const S2*)d->psi + base0;
+  T2 z[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int f = tid + q * NTHREADS;
+    z[q].x = 0;
+    z[q].y = 0;
+    if (live(f)) z[q] = cvt2<T>(gld(&psi_in[goff(f)]));
+  }
+  T2 pf[PF1][EPT];
+  if constexpr (PFN > 0) {
+#pragma unroll
+    for (int u = 0; u < PFN; ++u) {
+      const T2* tab = (const T2*)d->pf_src[u] + base0;
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+        const int f = tid + q * NTHREADS;
+        pf[u][q].x = 0;
+        pf[u][q].y = 0;
+        if (live(f)) pf[u][q] = gld(&tab[goff(f)]);
+      }
+    }
+  }
+  setup_lanes<T>(d, lt, tile, tid);
+  stage_lds_arrays(d, smem, tid);
+  __syncthreads();
+  if (inherit) run_inherited(d, tid);
+  Acc acc;
+  // one stage over the elements of this thread, FCH at a time; `src` 0 takes
+  // them from z (first stage), 1 from LDS; `dst` 0 writes LDS, 1 stores psi
+  auto stage_chunks = [&](const Stage& st, int src, int dst) {
+#pragma unroll
+    for (int c = 0; c < EPT; c += FCH) {
+      int ql[FCH], qk[FCH];
+      T2 zc[FCH];
+      T2 pc[PF1][FCH];
+#pragma unroll
+      for (int q = 0; q < FCH; ++q) {
+        const int f = tid + (c + q) * NTHREADS;
+        ql[q] = f < tot ? lane_of(f) : 0;
+        qk[q] = f < tot ? idx_of(f) : 0;
+        if (src == 0) {
+          zc[q] = z[c + q];
+        } else {
+          zc[q].x = 0;
+          zc[q].y = 0;
+          if (f < tot) zc[q] = lld(&cur[ql[q] * ld + fpad(qk[q])]);
+        }
+#pragma unroll
+        for (int u = 0; u < PF1; ++u) pc[u][q] = pf[u][c + q];
+      }
+      if (st.n > 0) stage_vec<T, FCH, PFN>(st, zc, ql, qk, tid + c * NTHREADS, tot, lt, d, acc, smem, pc);
+#pragma unroll
+      for (int q = 0; q < FCH; ++q) {
+        const int f = tid + (c + q) * NTHREADS;
+        if (dst == 0) {
+          if (f < tot) lst(&cur[ql[q] * ld + fpad(qk[q])], zc[q]);
+        } else {
+          if (live(f)) gst(&((S2*)d->psi)[base0 + goff(f)], cvt2<S>(zc[q]));
+        }
+      }
+    }
+  };
+  stage_chunks(d->st[0], 0, 0);
+  __syncthreads();
+  const int nx = d->nx;
+  for (int xi = 0; xi < nx; ++xi) {
+    const XformDesc& x = d->xf[xi];
+    if (xi == 1 && d->st[1].n > 0) {
+      stage_chunks(d->st[1], 1, 0);
+      __syncthreads();
+    }
+    fft_lds<EPT, T, true>(cur, tl, n, ld, twl, x.scale, x.inverse != 0, x.nrad, x.rad, tid, tl >= NTHREADS / 64);
+  }
+  stage_chunks(d->st[nx], 1, 1);
+  finish_pass(d, acc, tid, tile, ntiles);
+}
+
+// FP32 tiles of 8 elements per thread are held to 168 VGPRs (3 waves per SIMD).
+template <typename S, typename T, int EPT, int PFN>
+__global__ __launch_bounds__(NTHREADS)
+__attribute__((amdgpu_waves_per_eu((sizeof(T) == 4 && EPT == 8 && PFN <= 1) ? 3 : 1))) void fft_lean_kernel(const PassDesc* __restrict__ d, int64_t ntiles,
+                                                            int inherit_block) {
+  const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
+  if (b >= ntiles) {
+    run_inherited(d, threadIdx.x);
+    return;
+  }
+  fft_lean_body<S, T, EPT, PFN>(d, b, ntiles, b == 0 && !inherit_block);
 }
 
 // One launch for the same pass position of many independent wave functions
@@ -1541,6 +1791,57 @@ static void set_lds_attr() {
                             (int)SOD_MAX_DYN_LDS);
 }
 
+// The lean FFT kernel takes every pass whose transforms are all FFTs of a
+// power-of-two length; SO_FFT_LEAN=0 in the environment selects the general
+// kernel instead.
+static bool fft_lean_enabled() {
+  static const int on = [] {
+    const char* e = getenv("SO_FFT_LEAN");
+    return (e && e[0] == '0') ? 0 : 1;
+  }();
+  return on != 0;
+}
+
+static bool fft_lean_fits(const PassDesc& h) {
+  if (!fft_lean_enabled() || h.nx == 0 || h.xf[0].type != XF_FFT || (h.nx == 2 && h.xf[1].type != XF_FFT))
+    return false;
+  const int64_t n = h.n;
+  if (n < 2 || (n & (n - 1)) != 0 || (h.tl & (h.tl - 1)) != 0) return false;
+  return h.n_pf <= (h.ept <= 8 ? MAX_PF : 0);
+}
+
+template <typename S, typename T, int EPT, int PFN>
+static void set_lds_attr_lean() {
+  (void)hipFuncSetAttribute((const void*)fft_lean_kernel<S, T, EPT, PFN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)SOD_MAX_DYN_LDS);
+}
+
+template <typename S, typename T>
+static void set_lds_attr_lean_all() {
+  set_lds_attr_lean<S, T, 4, 0>();
+  set_lds_attr_lean<S, T, 4, 1>();
+  set_lds_attr_lean<S, T, 4, 2>();
+  set_lds_attr_lean<S, T, 8, 0>();
+  set_lds_attr_lean<S, T, 8, 1>();
+  set_lds_attr_lean<S, T, 8, 2>();
+  set_lds_attr_lean<S, T, 16, 0>();
+}
+
+template <typename S, typename T>
+static void launch_fft_lean(const PassDesc* d_desc, const PassDesc& h, dim3 grid, size_t lds, int64_t nt, int ib,
+                            hipStream_t s) {
+  const int key = h.ept * 4 + h.n_pf;
+  switch (key) {
+    case 16: fft_lean_kernel<S, T, 4, 0><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
+    case 17: fft_lean_kernel<S, T, 4, 1><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
+    case 18: fft_lean_kernel<S, T, 4, 2><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
+    case 32: fft_lean_kernel<S, T, 8, 0><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
+    case 33: fft_lean_kernel<S, T, 8, 1><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
+    case 34: fft_lean_kernel<S, T, 8, 2><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
+    default: fft_lean_kernel<S, T, 16, 0><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
+  }
+}
+
 template <typename S, typename T>
 static void launch_pass_st(const PassDesc* d_desc, const PassDesc& h, dim3 grid, size_t lds, int64_t nt, int ib,
                            hipStream_t s) {
@@ -1548,6 +1849,8 @@ static void launch_pass_st(const PassDesc* d_desc, const PassDesc& h, dim3 grid,
     elem_pass_kernel<S, T, SOD_ELEM_EPT><<<grid, NTHREADS, 0, s>>>(d_desc, nt, ib);
   else if (h.xf[0].type == XF_MAT)
     launch_xform<S, T, XF_MAT>(d_desc, h.ept, grid, lds, nt, ib, s);
+  else if (fft_lean_fits(h))
+    launch_fft_lean<S, T>(d_desc, h, grid, lds, nt, ib, s);
   else
     launch_xform<S, T, XF_FFT>(d_desc, h.ept, grid, lds, nt, ib, s);
 }
@@ -1567,6 +1870,9 @@ int sod_launch_pass(int dtype, const PassDesc* d_desc, const PassDesc& h, hipStr
     set_lds_attr<double, double, XF_MAT>();
     set_lds_attr<float, double, XF_FFT>();
     set_lds_attr<float, double, XF_MAT>();
+    set_lds_attr_lean_all<float, float>();
+    set_lds_attr_lean_all<double, double>();
+    set_lds_attr_lean_all<float, double>();
     attr_set = true;
   }
   if (lds > SOD_MAX_DYN_LDS) return (int)hipErrorInvalidValue;
