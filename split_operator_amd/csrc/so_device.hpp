@@ -23,7 +23,8 @@ constexpr int MAX_TL = 64;      // lanes per workgroup tile
 constexpr int NTHREADS = 256;   // threads per workgroup (4 wave64)
 constexpr int MAX_LDS_ARR = 4;  // small per-pass arrays staged in LDS at kernel start
 constexpr int MAX_PF = 2;       // full-grid tables prefetched into registers at kernel start
-constexpr int MAX_LF = 4;       // 1-D tables off the lane axis, read once per lane at kernel start
+constexpr int MAX_LF = 4;       // per-lane factors (products of 1-D tables off the lane axis), read once per lane at kernel start
+constexpr int MAX_LFK = 4;      // 1-D tables multiplied into one per-lane factor
 constexpr int MAX_HS = 4;       // scalar slots (EK_SCALE factors) read once at kernel start
 constexpr int FFT_EPT = 16;     // max complex elements per thread of a transform tile (tl * n <= FFT_EPT * NTHREADS)
 #define SOD_MAX_DYN_LDS (152 * 1024)  // dynamic LDS bytes a pass may request per workgroup (gfx950: 160 KiB)
@@ -47,7 +48,7 @@ struct ElemOp {
   const double* w_axis;  // EK_NORM: weights of the pass axis for the grid current at that point
   int32_t lds;    // >= 0: the table (EK_MUL1D on the pass axis) or weights (EK_NORM) are staged in LDS array `lds`
   int32_t pf;     // >= 0: EK_MULND values prefetched into register set `pf` at kernel start
-  int32_t lf;     // >= 0: EK_MUL1D off the lane axis: per-lane factor set `lf`; EK_SCALE: hoisted scalar `lf`
+  int32_t lf;     // >= 0: EK_MUL1D off the lane axis: per-lane factor `lf` (may stand for several such ops); EK_SCALE: hoisted scalar `lf`
   int32_t pad;
 };
 
@@ -111,7 +112,7 @@ struct PassDesc {
   int32_t nx;        // number of transforms (0, 1, 2)
   int32_t nacc;      // number of norm accumulators in use
   int32_t ept;       // elements per thread (transform passes: 4, 8, 16 or 32)
-  int32_t pad4;
+  int32_t fft_kernel;  // FFT passes: 0 = general pass kernel, 1 = lean kernel (power-of-two lanes)
   XformDesc xf[2];
   Stage st[3];       // st[0] before xf[0]; st[1] after xf[0]; st[2] after xf[1]
   const double* w_other[MAXD];  // weights of the other axes (grids fixed during the pass)
@@ -135,8 +136,9 @@ struct PassDesc {
   // 1-D tables indexed by an axis other than the lane axis: one factor per lane, read at kernel start
   int32_t n_lf;
   int32_t n_hs;
-  const void* lf_src[MAX_LF];
-  int32_t lf_axis[MAX_LF];
+  const void* lf_src[MAX_LF][MAX_LFK];  // factor j = prod_m lf_src[j][m][index along lf_axis[j][m]]
+  int32_t lf_axis[MAX_LF][MAX_LFK];
+  int32_t lf_cnt[MAX_LF];
   int32_t hs_slot[MAX_HS];  // scalar slots the EK_SCALE ops read (final before the launch)
   // scalar program inherited from the previous pass (run by workgroup 0 at
   // start, or by one extra workgroup when inherit_block is set)

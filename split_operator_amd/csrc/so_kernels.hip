@@ -765,19 +765,20 @@ __device__ __forceinline__ void mat_lds(const T2* in, T2* out, int tl, int ld, c
 }
 
 // ---------------------------------------------------------------- lane bookkeeping
+template <typename T>
 struct LaneTab {
   int64_t base[MAX_TL];
   double w[MAX_TL];
   int32_t idx[MAX_TL][MAXD];
   int32_t valid[MAX_TL];
-  double lf[MAX_LF][MAX_TL][2];  // per-lane factors of 1-D tables off the lane axis
-  double hs[MAX_HS];             // EK_SCALE factors
+  typename CT<T>::T2 lf[MAX_LF][MAX_TL];  // per-lane factors of 1-D tables off the lane axis
+  T hs[MAX_HS];                           // EK_SCALE factors, in the arithmetic precision
 };
 
 template <typename T>
-__device__ __forceinline__ void setup_lanes(const PassDesc* __restrict__ d, LaneTab& lt, int64_t tile, int tid) {
+__device__ __forceinline__ void setup_lanes(const PassDesc* __restrict__ d, LaneTab<T>& lt, int64_t tile, int tid) {
   using T2 = typename CT<T>::T2;
-  if (tid >= 64 && tid < 64 + d->n_hs) lt.hs[tid - 64] = gld(&d->S[d->hs_slot[tid - 64]]);
+  if (tid >= 64 && tid < 64 + d->n_hs) lt.hs[tid - 64] = (T)gld(&d->S[d->hs_slot[tid - 64]]);
   const int tl = d->tl;
   if (tid < tl) {
     const int l = tid;
@@ -818,9 +819,9 @@ __device__ __forceinline__ void setup_lanes(const PassDesc* __restrict__ d, Lane
     }
     lt.w[l] = w;
     for (int j = 0; j < d->n_lf; ++j) {
-      const T2 f = gld(&((const T2*)d->lf_src[j])[lt.idx[l][d->lf_axis[j]]]);
-      lt.lf[j][l][0] = (double)f.x;
-      lt.lf[j][l][1] = (double)f.y;
+      T2 f = gld(&((const T2*)d->lf_src[j][0])[lt.idx[l][d->lf_axis[j][0]]]);
+      for (int m = 1; m < d->lf_cnt[j]; ++m) f = cmul(f, gld(&((const T2*)d->lf_src[j][m])[lt.idx[l][d->lf_axis[j][m]]]));
+      lt.lf[j][l] = f;
     }
   }
 }
@@ -845,7 +846,7 @@ struct Acc {
 // full-grid tables prefetched into registers (`e.pf`) are read from there.
 template <typename T, int EPT, int PFN, typename T2>
 __device__ __forceinline__ void stage_vec(const Stage& st, T2 (&z)[EPT], const int (&ql)[EPT], const int (&qk)[EPT],
-                                          int tid, int tot, const LaneTab& lt, const PassDesc* __restrict__ d,
+                                          int tid, int tot, const LaneTab<T>& lt, const PassDesc* __restrict__ d,
                                           Acc& acc, const unsigned char* lds_base, const T2 (&pf)[PFN > 0 ? PFN : 1][EPT]) {
   const int nops = st.n;
   const int axis = d->axis;
@@ -864,10 +865,7 @@ __device__ __forceinline__ void stage_vec(const Stage& st, T2 (&z)[EPT], const i
 #pragma unroll
         for (int q = 0; q < EPT; ++q)
           if (tid + q * NTHREADS < tot) {
-            T2 f;
-            f.x = (T)lt.lf[e.lf][ql[q]][0];
-            f.y = (T)lt.lf[e.lf][ql[q]][1];
-            z[q] = cmul(z[q], f);
+            z[q] = cmul(z[q], lt.lf[e.lf][ql[q]]);
           }
       } else {
         const T2* tab = (const T2*)e.table;
@@ -897,7 +895,7 @@ __device__ __forceinline__ void stage_vec(const Stage& st, T2 (&z)[EPT], const i
 #pragma unroll
       for (int q = 0; q < EPT; ++q) z[q] = cmul(z[q], c);
     } else if (kind == EK_SCALE) {
-      const T sc = e.lf >= 0 ? (T)lt.hs[e.lf] : (T)gld(&d->S[e.slot]);
+      const T sc = e.lf >= 0 ? lt.hs[e.lf] : (T)gld(&d->S[e.slot]);
 #pragma unroll
       for (int q = 0; q < EPT; ++q) {
         z[q].x *= sc;
@@ -1262,7 +1260,7 @@ __device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, 
   using S2 = typename CT<S>::T2;
   constexpr int PFN = EPT <= 8 ? MAX_PF : 0;
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ LaneTab lt;
+  __shared__ LaneTab<T> lt;
   const int tid = threadIdx.x;
   const int tl = d->tl;
   const int n = (int)d->n;
@@ -1381,7 +1379,7 @@ __device__ __forceinline__ void fft_lean_body(const PassDesc* __restrict__ d, in
   using S2 = typename CT<S>::T2;
   constexpr int PF1 = PFN > 0 ? PFN : 1;
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ LaneTab lt;
+  __shared__ LaneTab<T> lt;
   const int tid = threadIdx.x;
   const int tl = d->tl;
   const int n = (int)d->n;
@@ -1391,6 +1389,13 @@ __device__ __forceinline__ void fft_lean_body(const PassDesc* __restrict__ d, in
   const bool strided = inner > 1;
   const int nsh = __builtin_ctz((unsigned)n);
   const int tsh = __builtin_ctz((unsigned)tl);
+  // element f of the tile: lane (f >> lsh) & (tl - 1), index (f >> ksh) & (n - 1),
+  // psi offset (f & gm) + (f >> gsh) * inner from the tile origin (uniform
+  // shifts and masks, so no per-element select between the two layouts)
+  const int lsh = strided ? 0 : nsh;
+  const int ksh = strided ? tsh : 0;
+  const int gm = strided ? tl - 1 : -1;
+  const int gsh = strided ? tsh : 31;
   int64_t base0;
   int nvalid;
   if (strided) {
@@ -1402,9 +1407,9 @@ __device__ __forceinline__ void fft_lean_body(const PassDesc* __restrict__ d, in
     base0 = tile * tl * (int64_t)n;
     nvalid = (int)min((int64_t)tl, d->outer - tile * tl);
   }
-  auto lane_of = [&](int f) { return strided ? (f & (tl - 1)) : (f >> nsh); };
-  auto idx_of = [&](int f) { return strided ? (f >> tsh) : (f & (n - 1)); };
-  auto goff = [&](int f) { return strided ? (int64_t)(f & (tl - 1)) + (int64_t)(f >> tsh) * inner : (int64_t)f; };
+  auto lane_of = [&](int f) { return (f >> lsh) & (tl - 1); };
+  auto idx_of = [&](int f) { return (f >> ksh) & (n - 1); };
+  auto goff = [&](int f) { return (int64_t)(f & gm) + (int64_t)(f >> gsh) * inner; };
   auto live = [&](int f) { return f < tot && lane_of(f) < nvalid; };
   T2* cur = (T2*)smem;
   const T2* twl = (const T2*)(smem + d->lds_arr_off[0]);
@@ -1791,9 +1796,9 @@ static void set_lds_attr() {
                             (int)SOD_MAX_DYN_LDS);
 }
 
-// The lean FFT kernel takes every pass whose transforms are all FFTs of a
-// power-of-two length; SO_FFT_LEAN=0 in the environment selects the general
-// kernel instead.
+// The lean FFT kernel takes the passes the host marked for it (fft_kernel 1)
+// whose transforms are all FFTs of a power-of-two length; SO_FFT_LEAN=0 in the
+// environment selects the general kernel for every pass.
 static bool fft_lean_enabled() {
   static const int on = [] {
     const char* e = getenv("SO_FFT_LEAN");
@@ -1803,7 +1808,8 @@ static bool fft_lean_enabled() {
 }
 
 static bool fft_lean_fits(const PassDesc& h) {
-  if (!fft_lean_enabled() || h.nx == 0 || h.xf[0].type != XF_FFT || (h.nx == 2 && h.xf[1].type != XF_FFT))
+  if (!fft_lean_enabled() || h.fft_kernel != 1 || h.nx == 0 || h.xf[0].type != XF_FFT ||
+      (h.nx == 2 && h.xf[1].type != XF_FFT))
     return false;
   const int64_t n = h.n;
   if (n < 2 || (n & (n - 1)) != 0 || (h.tl & (h.tl - 1)) != 0) return false;

@@ -772,7 +772,7 @@ struct Propagation {
 
   // ----------------------------------------------------------- compile
   CompiledPass* compile(const PassB& b, bool immediate);
-  std::unique_ptr<CompiledPass> build(const PassB& b, bool immediate, int force_tl);
+  std::unique_ptr<CompiledPass> build(const PassB& b, bool immediate, int force_tl, int force_kernel = -1);
   int tune_tile(const PassB& b, bool immediate);
   bool autotune = std::getenv("SO_AUTOTUNE") != nullptr && std::getenv("SO_AUTOTUNE")[0] == '1';
   void* tune_scratch = nullptr;
@@ -933,17 +933,19 @@ CompiledPass* Propagation::compile(const PassB& b, bool immediate) {
     pending_prog = (c->h.has_norm && c->h.fin_mode == sod::FIN_DEFERRED) ? c : nullptr;
     return c;
   }
-  const int tl = (autotune && b.nx > 0) ? tune_tile(b, immediate) : 0;
-  std::unique_ptr<CompiledPass> cp = build(b, immediate, tl);
+  // tuned choice: tile lanes in the low 16 bits, FFT kernel + 1 above
+  const int tv = (autotune && b.nx > 0) ? tune_tile(b, immediate) : 0;
+  std::unique_ptr<CompiledPass> cp = build(b, immediate, tv & 0xffff, tv ? (tv >> 16) - 1 : -1);
   CompiledPass* raw = cp.get();
   cache.emplace(std::move(key), std::move(cp));
   pending_prog = (raw->h.has_norm && raw->h.fin_mode == sod::FIN_DEFERRED) ? raw : nullptr;
   return raw;
 }
 
-// The descriptor of pass b with tile size force_tl (0: the rule below). Pure:
-// no cache or pending-program bookkeeping.
-std::unique_ptr<CompiledPass> Propagation::build(const PassB& b, bool immediate, int force_tl) {
+// The descriptor of pass b with tile size force_tl (0: the rule below) and FFT
+// kernel force_kernel (-1: the rule below). Pure: no cache or pending-program
+// bookkeeping.
+std::unique_ptr<CompiledPass> Propagation::build(const PassB& b, bool immediate, int force_tl, int force_kernel) {
   const bool scalar_only = b.nx == 0 && b.st[0].empty();
   auto cp = std::make_unique<CompiledPass>();
   sod::PassDesc& h = cp->h;
@@ -1048,6 +1050,11 @@ std::unique_ptr<CompiledPass> Propagation::build(const PassB& b, bool immediate,
       h.ept = tot <= 4 * sod::NTHREADS ? 4 : tot <= 8 * sod::NTHREADS ? 8 : 16;
     }
     h.tl = tl;
+    // FFT-only passes: the lean kernel when every wave of the workgroup owns
+    // whole lanes (throughput-bound tiles); single-lane tiles of long lanes
+    // are latency-bound and keep the general kernel
+    h.fft_kernel = 0;
+    if (b.nx > 0 && !has_mat) h.fft_kernel = force_kernel >= 0 ? force_kernel : (tl >= 4 ? 1 : 0);
     if (b.nx == 0) {
       int64_t size = 1;
       for (int a = 0; a < wf->ndim; ++a) size *= wf->shape[a];
@@ -1116,14 +1123,61 @@ std::unique_ptr<CompiledPass> Propagation::build(const PassB& b, bool immediate,
       };
       // 1-D tables off the lane axis give one factor per lane; EK_SCALE
       // factors are final before the launch: both are read once at kernel start
-      auto add_lf = [&](const void* src, int ax) -> int {
-        for (int i = 0; i < h.n_lf; ++i)
-          if (h.lf_src[i] == src && h.lf_axis[i] == ax) return i;
-        if (h.n_lf >= sod::MAX_LF) return -1;
-        h.lf_src[h.n_lf] = src;
-        h.lf_axis[h.n_lf] = ax;
-        return h.n_lf++;
+      using LfGroup = std::vector<std::pair<const void*, int>>;
+      auto add_lf = [&](const LfGroup& g) -> int {
+        for (int i = 0; i < h.n_lf; ++i) {
+          bool same = h.lf_cnt[i] == (int)g.size();
+          for (size_t m = 0; same && m < g.size(); ++m)
+            same = h.lf_src[i][m] == g[m].first && h.lf_axis[i][m] == g[m].second;
+          if (same) return i;
+        }
+        if (h.n_lf >= sod::MAX_LF || g.size() > (size_t)sod::MAX_LFK) return -1;
+        const int i = h.n_lf++;
+        h.lf_cnt[i] = (int)g.size();
+        for (size_t m = 0; m < g.size(); ++m) {
+          h.lf_src[i][m] = g[m].first;
+          h.lf_axis[i][m] = g[m].second;
+        }
+        return i;
       };
+      // The off-axis 1-D tables of a run of ops with no norm between them
+      // multiply into one per-lane factor (the ops of a stage are products,
+      // so they commute up to rounding; a norm must see the factors applied
+      // before it). The first op of the run applies the product and the
+      // others are dropped. Runs that do not fit the factor slots stay one
+      // op per table.
+      for (int st = 0; st < 3; ++st) {
+        sod::Stage& S = h.st[st];
+        std::vector<std::pair<int, LfGroup>> runs;  // (first op, tables)
+        int open_run = -1;
+        for (int i = 0; i < S.n; ++i) {
+          const sod::ElemOp& o = S.op[i];
+          if (o.kind == sod::EK_NORM) open_run = -1;
+          if (o.kind == sod::EK_MUL1D && o.axis != axis) {
+            if (open_run < 0 || runs[open_run].second.size() >= (size_t)sod::MAX_LFK) {
+              runs.push_back({i, {}});
+              open_run = (int)runs.size() - 1;
+            }
+            runs[open_run].second.push_back({o.table, o.axis});
+          }
+        }
+        std::vector<int> drop(S.n, 0);
+        for (auto& r : runs) {
+          if (r.second.size() < 2) continue;
+          const int j = add_lf(r.second);
+          if (j < 0) continue;
+          S.op[r.first].lf = j;
+          for (int i = r.first + 1, m = 1; i < S.n && m < (int)r.second.size(); ++i)
+            if (S.op[i].kind == sod::EK_MUL1D && S.op[i].axis != axis && S.op[i].table == r.second[m].first) {
+              drop[i] = 1;
+              ++m;
+            }
+        }
+        int out = 0;
+        for (int i = 0; i < S.n; ++i)
+          if (!drop[i]) S.op[out++] = S.op[i];
+        S.n = out;
+      }
       auto add_hs = [&](int slot) -> int {
         for (int i = 0; i < h.n_hs; ++i)
           if (h.hs_slot[i] == slot) return i;
@@ -1135,7 +1189,7 @@ std::unique_ptr<CompiledPass> Propagation::build(const PassB& b, bool immediate,
         for (int i = 0; i < h.st[st].n; ++i) {
           sod::ElemOp& o = h.st[st].op[i];
           if (o.kind == sod::EK_MUL1D && o.axis == axis) o.lds = add_arr(o.table, n * (int64_t)cs);
-          if (o.kind == sod::EK_MUL1D && o.axis != axis) o.lf = add_lf(o.table, o.axis);
+          if (o.kind == sod::EK_MUL1D && o.axis != axis && o.lf < 0) o.lf = add_lf({{o.table, o.axis}});
           if (o.kind == sod::EK_SCALE) o.lf = add_hs(o.slot);
           if (o.kind == sod::EK_NORM) o.lds = add_arr(o.w_axis, n * 8);
           if (o.kind == sod::EK_MULND) o.pf = add_pf(o.table);
@@ -1168,8 +1222,11 @@ std::unique_ptr<CompiledPass> Propagation::build(const PassB& b, bool immediate,
 // Tile sizes chosen by measurement: for a transform pass, tile sizes from a
 // quarter to four times the rule's choice (those that fit) are each launched
 // on a scratch copy of psi with the scalar bookkeeping switched off, and the
-// fastest median of 7 launches wins. Choices are kept per pass shape for the
-// process (and in the file named by SO_TUNE_CACHE, if set).
+// fastest median of 7 launches wins; FFT-only passes of a power-of-two length
+// try each tile with both FFT kernels. The choice is stored as an int: tile
+// lanes in the low 16 bits, FFT kernel + 1 above (0: the rule's kernel).
+// Choices are kept per pass shape for the process (and in the file named by
+// SO_TUNE_CACHE, if set).
 static std::map<std::string, int>& tune_table() {
   static std::map<std::string, int> t;
   static bool loaded = false;
@@ -1210,7 +1267,7 @@ int Propagation::tune_tile(const PassB& b, bool immediate) {
   int64_t outer = 1, inner = 1;
   for (int a = 0; a < axis; ++a) outer *= wf->shape[a];
   for (int a = axis + 1; a < wf->ndim; ++a) inner *= wf->shape[a];
-  std::snprintf(kb, sizeof(kb), "d%d-b%d-g%d-n%lld-o%lld-i%lld-x%d-%d-%d-s%zu-%zu-%zu", dtype(), batch_mode ? 1 : 0,
+  std::snprintf(kb, sizeof(kb), "k-d%d-b%d-g%d-n%lld-o%lld-i%lld-x%d-%d-%d-s%zu-%zu-%zu", dtype(), batch_mode ? 1 : 0,
                 gemm_mode, (long long)wf->shape[axis], (long long)outer, (long long)inner, b.nx, b.xf[0].type,
                 b.nx > 1 ? b.xf[1].type : 0, b.st[0].size(), b.st[1].size(), b.st[2].size());
   const std::string key = kb;
@@ -1219,6 +1276,10 @@ int Propagation::tune_tile(const PassB& b, bool immediate) {
   if (it != tt.end()) return it->second;
   std::unique_ptr<CompiledPass> base = build(b, immediate, 0);
   const int tl0 = base->h.tl;
+  // FFT-only passes of a power-of-two length also try both FFT kernels
+  const int64_t n = wf->shape[axis];
+  const bool fft_only = b.xf[0].type == sod::XF_FFT && (b.nx < 2 || b.xf[1].type == sod::XF_FFT);
+  const int nk = (fft_only && (n & (n - 1)) == 0 && !batch_mode) ? 2 : 1;
   release_pass(ctx, base.get());
   if (!tune_scratch) tune_scratch = ctx->alloc(csize(dtype()) * wf->size);
   HIPCHK(hipMemcpyAsync(tune_scratch, wf->data, csize(dtype()) * wf->size, hipMemcpyDeviceToDevice, ctx->stream));
@@ -1227,10 +1288,13 @@ int Propagation::tune_tile(const PassB& b, bool immediate) {
   HIPCHK(hipEventCreate(&e1));
   int best = tl0;
   float best_ms = 1e30f;
-  for (int t = std::max(1, tl0 / 4); t <= tl0 * 4; t *= 2) {
+  for (int cand = 0; cand < 6 * nk; ++cand) {
+    const int t = (std::max(1, tl0 / 4)) << (cand / nk);
+    const int kern = nk == 2 ? cand % nk : -1;
+    if (t > tl0 * 4) break;
     std::unique_ptr<CompiledPass> c;
     try {
-      c = build(b, immediate, t);
+      c = build(b, immediate, t, kern);
     } catch (const SoError&) {
       continue;  // does not fit
     }
@@ -1256,7 +1320,7 @@ int Propagation::tune_tile(const PassB& b, bool immediate) {
     const float med = ms[ms.size() / 2];
     if (med < best_ms) {
       best_ms = med;
-      best = t;
+      best = t | (kern >= 0 ? (kern + 1) << 16 : 0);
     }
     release_pass(ctx, c.get());
   }
