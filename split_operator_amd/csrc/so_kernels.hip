@@ -1407,33 +1407,56 @@ __device__ __forceinline__ void fft_lean_body(const PassDesc* __restrict__ d, in
     base0 = tile * tl * (int64_t)n;
     nvalid = (int)min((int64_t)tl, d->outer - tile * tl);
   }
+  // byte offsets from the tile origin fit 32 bits (the host routes larger
+  // grids to the general kernel), so loads and stores take a uniform base
+  // plus a 32-bit offset
+  const uint32_t inner32 = (uint32_t)inner;
   auto lane_of = [&](int f) { return (f >> lsh) & (tl - 1); };
   auto idx_of = [&](int f) { return (f >> ksh) & (n - 1); };
-  auto goff = [&](int f) { return (int64_t)(f & gm) + (int64_t)(f >> gsh) * inner; };
+  auto goff = [&](int f) { return (uint32_t)(f & gm) + (uint32_t)(f >> gsh) * inner32; };
+  auto at = [&](const void* base, int f, uint32_t esize) -> const void* {
+    return (const char*)base + goff(f) * esize;
+  };
+  // full tile: every element of every thread is a valid lane element
+  const bool full = nvalid == tl && tot == EPT * NTHREADS;
   auto live = [&](int f) { return f < tot && lane_of(f) < nvalid; };
   T2* cur = (T2*)smem;
   const T2* twl = (const T2*)(smem + d->lds_arr_off[0]);
   // every global read of the tile is issued first
   const S2* psi_in = (const S2*)d->psi + base0;
   T2 z[EPT];
-#pragma unroll
-  for (int q = 0; q < EPT; ++q) {
-    const int f = tid + q * NTHREADS;
-    z[q].x = 0;
-    z[q].y = 0;
-    if (live(f)) z[q] = cvt2<T>(gld(&psi_in[goff(f)]));
-  }
   T2 pf[PF1][EPT];
-  if constexpr (PFN > 0) {
+  if (full) {
 #pragma unroll
-    for (int u = 0; u < PFN; ++u) {
-      const T2* tab = (const T2*)d->pf_src[u] + base0;
+    for (int q = 0; q < EPT; ++q)
+      z[q] = cvt2<T>(gld((const S2*)at(psi_in, tid + q * NTHREADS, sizeof(S2))));
+    if constexpr (PFN > 0) {
 #pragma unroll
-      for (int q = 0; q < EPT; ++q) {
-        const int f = tid + q * NTHREADS;
-        pf[u][q].x = 0;
-        pf[u][q].y = 0;
-        if (live(f)) pf[u][q] = gld(&tab[goff(f)]);
+      for (int u = 0; u < PFN; ++u) {
+        const T2* tab = (const T2*)d->pf_src[u] + base0;
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) pf[u][q] = gld((const T2*)at(tab, tid + q * NTHREADS, sizeof(T2)));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      const int f = tid + q * NTHREADS;
+      z[q].x = 0;
+      z[q].y = 0;
+      if (live(f)) z[q] = cvt2<T>(gld((const S2*)at(psi_in, f, sizeof(S2))));
+    }
+    if constexpr (PFN > 0) {
+#pragma unroll
+      for (int u = 0; u < PFN; ++u) {
+        const T2* tab = (const T2*)d->pf_src[u] + base0;
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+          const int f = tid + q * NTHREADS;
+          pf[u][q].x = 0;
+          pf[u][q].y = 0;
+          if (live(f)) pf[u][q] = gld((const T2*)at(tab, f, sizeof(T2)));
+        }
       }
     }
   }
@@ -1442,6 +1465,16 @@ __device__ __forceinline__ void fft_lean_body(const PassDesc* __restrict__ d, in
   __syncthreads();
   if (inherit) run_inherited(d, tid);
   Acc acc;
+  S2* psi_out = (S2*)d->psi + base0;
+  // LDS index of element q of this thread. With contiguous lanes of n >= 32
+  // and the pitch ld = n + n/32, lane l index k sits at fpad(l n + k) =
+  // fpad(f), so element q is at fpad(tid) + q (NTHREADS + NTHREADS/32): a
+  // compile-time step. Other tiles compute it per element.
+  const bool seam = !strided && n >= 32 && ld == n + (n >> 5);
+  const int off0 = fpad(tid);
+  auto lds_at = [&](int q, int f) {
+    return seam ? off0 + q * (NTHREADS + NTHREADS / 32) : lane_of(f) * ld + fpad(idx_of(f));
+  };
   // one stage over the elements of this thread, FCH at a time; `src` 0 takes
   // them from z (first stage), 1 from LDS; `dst` 0 writes LDS, 1 stores psi
   auto stage_chunks = [&](const Stage& st, int src, int dst) {
@@ -1460,7 +1493,7 @@ __device__ __forceinline__ void fft_lean_body(const PassDesc* __restrict__ d, in
         } else {
           zc[q].x = 0;
           zc[q].y = 0;
-          if (f < tot) zc[q] = lld(&cur[ql[q] * ld + fpad(qk[q])]);
+          if (f < tot) zc[q] = lld(&cur[lds_at(c + q, f)]);
         }
 #pragma unroll
         for (int u = 0; u < PF1; ++u) pc[u][q] = pf[u][c + q];
@@ -1470,9 +1503,9 @@ __device__ __forceinline__ void fft_lean_body(const PassDesc* __restrict__ d, in
       for (int q = 0; q < FCH; ++q) {
         const int f = tid + (c + q) * NTHREADS;
         if (dst == 0) {
-          if (f < tot) lst(&cur[ql[q] * ld + fpad(qk[q])], zc[q]);
+          if (f < tot) lst(&cur[lds_at(c + q, f)], zc[q]);
         } else {
-          if (live(f)) gst(&((S2*)d->psi)[base0 + goff(f)], cvt2<S>(zc[q]));
+          if (full || live(f)) gst((S2*)at(psi_out, f, sizeof(S2)), cvt2<S>(zc[q]));
         }
       }
     }
@@ -1813,6 +1846,9 @@ static bool fft_lean_fits(const PassDesc& h) {
     return false;
   const int64_t n = h.n;
   if (n < 2 || (n & (n - 1)) != 0 || (h.tl & (h.tl - 1)) != 0) return false;
+  // byte offsets inside a tile (up to n * inner elements of at most 16 bytes)
+  // must fit 32 bits
+  if (n * h.inner >= ((int64_t)1 << 28)) return false;
   return h.n_pf <= (h.ept <= 8 ? MAX_PF : 0);
 }
 

@@ -918,8 +918,13 @@ static const void* fft_twiddles(so_op* op, int dtype) {
 
 // LDS row pitch (complex elements) of a pass tile: FFT rows hold padded
 // indices (one pad per 32); matrix rows are spaced 8 banks apart (pitch = 4 mod 32).
-static int64_t lds_pitch(bool mat, int64_t n, int64_t tl) {
+static int64_t lds_pitch(bool mat, int64_t n, int64_t tl, bool contiguous) {
   if (mat) return n + ((4 - n) % 32 + 32) % 32;
+  // FFT tiles: one pad slot per 32 elements. Contiguous lanes that are whole
+  // pad groups continue the padding across rows (lane l element k at
+  // fpad(l n + k), the tile's memory order); strided tiles, whose threads
+  // step across lanes, and ragged lanes get one extra slot per row
+  if (contiguous && n % 32 == 0) return n + (n >> 5);
   return n + (n >> 5) + (tl > 1 ? 1 : 0);
 }
 
@@ -1025,7 +1030,7 @@ std::unique_ptr<CompiledPass> Propagation::build(const PassB& b, bool immediate,
       // there too when they fit (checked when the descriptor is filled)
       const int64_t tw_bytes = mat ? 0 : (n * (int64_t)cs + 15) / 16 * 16;
       const int bufs = mat ? 2 : 1;
-      auto lds_for = [&](int64_t t) { return (int64_t)bufs * t * lds_pitch(mat, n, t) * (int64_t)cs + tw_bytes; };
+      auto lds_for = [&](int64_t t) { return (int64_t)bufs * t * lds_pitch(mat, n, t, h.inner == 1) * (int64_t)cs + tw_bytes; };
       int64_t cap = std::min<int64_t>(sod::MAX_TL, lanes_avail);
       while (cap > 1 && (lds_for(cap) > SOD_PREF_DYN_LDS || cap * n > (int64_t)sod::FFT_EPT * sod::NTHREADS)) cap /= 2;
       tl = (int)std::max<int64_t>(cap, 1);
@@ -1045,7 +1050,7 @@ std::unique_ptr<CompiledPass> Propagation::build(const PassB& b, bool immediate,
       }
       if ((int64_t)tl * n > (int64_t)sod::FFT_EPT * sod::NTHREADS || lds_for(tl) > SOD_MAX_DYN_LDS)
         fail(SO_ERR_UNSUPPORTED, "lane length %lld exceeds the LDS tile limit for this dtype", (long long)n);
-      h.ld = (int)lds_pitch(mat, n, tl);
+      h.ld = (int)lds_pitch(mat, n, tl, h.inner == 1);
       const int64_t tot = (int64_t)tl * n;
       h.ept = tot <= 4 * sod::NTHREADS ? 4 : tot <= 8 * sod::NTHREADS ? 8 : 16;
     }
