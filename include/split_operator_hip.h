@@ -212,7 +212,7 @@ typedef struct {
     int32_t xform_type;   /* 0 none, 1 FFT, 2 matrix */
     int32_t axis;
     int32_t lanes_per_tile;
-    int32_t nd_tables;    /* full-grid operator tables read */
+    int32_t nd_tables;    /* distinct full-grid operator tables read */
     int32_t norms;        /* FP64 norm reductions */
     int32_t line_ops;     /* one-dimensional tables and scale factors */
 } so_pass_timing;

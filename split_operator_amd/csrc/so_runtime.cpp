@@ -2731,10 +2731,14 @@ extern "C" int so_propagation_time_passes(so_propagation* pr, int64_t iters, int
       t.lanes = h.outer * h.inner;
       t.tiles = h.ntiles;
       t.lanes_per_tile = h.tl;
+      std::vector<const void*> nd;  // distinct full-grid tables (each is read once per launch)
       for (int s = 0; s < 3; ++s)
         for (int i = 0; i < h.st[s].n; ++i) {
           const int k = h.st[s].op[i].kind;
-          if (k == sod::EK_MULND) t.nd_tables++;
+          if (k == sod::EK_MULND && std::find(nd.begin(), nd.end(), h.st[s].op[i].table) == nd.end()) {
+            nd.push_back(h.st[s].op[i].table);
+            t.nd_tables++;
+          }
           if (k == sod::EK_NORM) t.norms++;
           if (k == sod::EK_MUL1D || k == sod::EK_MULB || k == sod::EK_SCALE) t.line_ops++;
         }
