@@ -46,6 +46,9 @@ enum { SO_COMPLEX_F32 = 0, SO_COMPLEX_F64 = 1, SO_COMPLEX_F32_F64_ARITH = 2 };
 /* lane GEMM of matrix transformations in FP32-arithmetic passes: FP32 MFMA, or each FP32 operand split into two
  * BF16 parts and the product formed as hi.hi + hi.lo + lo.hi on BF16 MFMA (FP32 sums, ~2^-16 relative; K <= 128) */
 enum { SO_GEMM_FP32 = 0, SO_GEMM_SPLIT_BF16X3 = 1 };
+/* FFT pass kernel: the step compiler's rule (and the autotuner, when on), the lean kernel wherever a pass qualifies
+ * (FFT-only, power-of-two lanes), or the general pass kernel */
+enum { SO_FFT_KERNEL_AUTO = 0, SO_FFT_KERNEL_LEAN = 1, SO_FFT_KERNEL_GENERAL = 2 };
 enum { SO_APPLY_NONE = 0, SO_APPLY_FIRST_HALF = 1, SO_APPLY_SECOND_HALF = 2 }; /* control.rs:19-26 */
 enum { SO_ORDER_NORMAL = 0, SO_ORDER_INVERSE_FIRST = 1 };        /* transformation.rs:13-17 */
 
@@ -178,6 +181,7 @@ int so_propagation_create(so_context *ctx, so_propagation **out);               
 int so_propagation_set_wave_function(so_propagation *p, so_wave_function *wf);           /* :91-93 (copies) */
 int so_propagation_set_time_grid(so_propagation *p, double step, int64_t step_no, int im_time); /* :96-98 */
 int so_propagation_set_gemm_mode(so_propagation *p, int mode); /* no crate counterpart; SO_GEMM_FP32 by default */
+int so_propagation_set_fft_kernel(so_propagation *p, int kernel); /* no crate counterpart; SO_FFT_KERNEL_AUTO by default */
 /* no crate counterpart: choose tile sizes of transform passes by timing candidates on a scratch copy of psi when a pass
  * is first compiled (choices cached per pass shape; SO_TUNE_CACHE names a file that keeps them; SO_AUTOTUNE=1 turns it
  * on by default) */

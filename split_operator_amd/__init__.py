@@ -548,6 +548,12 @@ class Propagation:
         self._ready()
         call("so_propagation_set_gemm_mode", self._h, {"fp32": 0, "bf16x3": 1}[mode])
 
+    def set_fft_kernel(self, kernel):
+        """FFT pass kernel: "auto" (the step compiler's rule, or the autotuner), "lean" (wherever a pass qualifies:
+        FFT-only, power-of-two lanes) or "general"."""
+        self._ready()
+        call("so_propagation_set_fft_kernel", self._h, {"auto": 0, "lean": 1, "general": 2}[kernel])
+
     def set_autotune(self, enable=True):
         """Choose transform-pass tile sizes by timing candidates when a pass is first compiled."""
         self._ready()

@@ -98,6 +98,7 @@ SIGNATURES = {
     "so_gauss_legendre": [_vp, C.c_int64, _dp, _dp],
     "so_propagation_set_gemm_mode": [_vp, C.c_int],
     "so_propagation_set_autotune": [_vp, C.c_int],
+    "so_propagation_set_fft_kernel": [_vp, C.c_int],
     "so_batch_create": [_vp, _vpp],
     "so_batch_add": [_vp, _vp],
     "so_batch_remove": [_vp, _vp],

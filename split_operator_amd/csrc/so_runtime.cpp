@@ -464,6 +464,7 @@ struct Propagation {
   // supplies the workgroups) rather than for filling the GPU on their own
   bool batch_mode = false;
   int gemm_mode = SO_GEMM_FP32;  // lane GEMM of FP32-arithmetic passes
+  int fft_kernel = SO_FFT_KERNEL_AUTO;  // FFT pass kernel choice
   struct GraphRec {
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
@@ -938,9 +939,12 @@ CompiledPass* Propagation::compile(const PassB& b, bool immediate) {
     pending_prog = (c->h.has_norm && c->h.fin_mode == sod::FIN_DEFERRED) ? c : nullptr;
     return c;
   }
-  // tuned choice: tile lanes in the low 16 bits, FFT kernel + 1 above
+  // tuned choice: tile lanes in the low 16 bits, FFT kernel + 1 above; a
+  // kernel set with so_propagation_set_fft_kernel overrides both
   const int tv = (autotune && b.nx > 0) ? tune_tile(b, immediate) : 0;
-  std::unique_ptr<CompiledPass> cp = build(b, immediate, tv & 0xffff, tv ? (tv >> 16) - 1 : -1);
+  int kern = tv ? (tv >> 16) - 1 : -1;
+  if (fft_kernel != SO_FFT_KERNEL_AUTO) kern = fft_kernel == SO_FFT_KERNEL_LEAN ? 1 : 0;
+  std::unique_ptr<CompiledPass> cp = build(b, immediate, tv & 0xffff, kern);
   CompiledPass* raw = cp.get();
   cache.emplace(std::move(key), std::move(cp));
   pending_prog = (raw->h.has_norm && raw->h.fin_mode == sod::FIN_DEFERRED) ? raw : nullptr;
@@ -1284,7 +1288,7 @@ int Propagation::tune_tile(const PassB& b, bool immediate) {
   // FFT-only passes of a power-of-two length also try both FFT kernels
   const int64_t n = wf->shape[axis];
   const bool fft_only = b.xf[0].type == sod::XF_FFT && (b.nx < 2 || b.xf[1].type == sod::XF_FFT);
-  const int nk = (fft_only && (n & (n - 1)) == 0 && !batch_mode) ? 2 : 1;
+  const int nk = (fft_only && (n & (n - 1)) == 0 && !batch_mode && fft_kernel == SO_FFT_KERNEL_AUTO) ? 2 : 1;
   release_pass(ctx, base.get());
   if (!tune_scratch) tune_scratch = ctx->alloc(csize(dtype()) * wf->size);
   HIPCHK(hipMemcpyAsync(tune_scratch, wf->data, csize(dtype()) * wf->size, hipMemcpyDeviceToDevice, ctx->stream));
@@ -2517,6 +2521,18 @@ extern "C" int so_propagation_set_gemm_mode(so_propagation* pr, int mode) {
   if (pr->p.gemm_mode != mode) {
     prop_reset_compiled(pr->p);
     pr->p.gemm_mode = mode;
+  }
+  API_END
+}
+
+extern "C" int so_propagation_set_fft_kernel(so_propagation* pr, int kernel) {
+  API_BEGIN
+  NEED(pr);
+  if (kernel != SO_FFT_KERNEL_AUTO && kernel != SO_FFT_KERNEL_LEAN && kernel != SO_FFT_KERNEL_GENERAL)
+    fail(SO_ERR_INVALID, "unknown FFT kernel %d", kernel);
+  if (pr->p.fft_kernel != kernel) {
+    prop_reset_compiled(pr->p);
+    pr->p.fft_kernel = kernel;
   }
   API_END
 }

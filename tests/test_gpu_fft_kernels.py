@@ -1,0 +1,150 @@
+"""The two FFT pass kernels (the general pass kernel and the lean kernel for
+power-of-two lanes) on the same stacks: each against the oracle, and against
+each other.
+
+The kernels share the stage interpreter, the transforms' formulas and the
+finalisation; they differ in tiling, addressing and register use, so the
+compiler contracts multiplies and adds into FMAs differently and the FP64 norm
+partials are summed in another order (chunks of four elements per thread in
+the lean kernel). The pair limits below are therefore round-off limits (an
+indexing mistake gives O(1) differences). The shapes exercise the lean
+kernel's edge cases: strided and contiguous lanes, tiles with fewer valid lanes
+than the tile holds, tiles smaller than the workgroup's element count, zero, one
+and two full-grid tables prefetched into registers, wave-local and
+workgroup-wide radix passes, norms and scalar programs (LeakControl, imaginary
+time), and all three precisions.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+so = pytest.importorskip("split_operator_amd")
+from oracle import oracle as O  # noqa: E402
+from split_operator_amd import workloads as W  # noqa: E402
+from tests import scenarios as S  # noqa: E402
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+@pytest.fixture(autouse=True)
+def _dtype_reset():
+    yield
+    so.set_default_dtype("f64")
+
+
+TOL_ORACLE = {"f64": 1e-10, "f32": 5e-5, "f32x": 3e-6}
+TOL_PAIR = {"f64": 1e-12, "f32": 1e-5, "f32x": 1e-6}
+
+
+def box_3d(B, shape, steps=20, dt=0.01, dtype=None, im_time=False):
+    """3-D box with per-axis lengths `shape` on [-8, 8): V = |x|^2 / 2 (a full-grid table), per-axis FFTs and
+    kinetic factors, Gaussian start with a momentum kick along every axis."""
+    grids = [B.Grid.new_linear_continuos("xyz"[a], -8.0, 8.0, n, a) for a, n in enumerate(shape)]
+    xs = [g.nodes for g in grids]
+    g1 = [np.array([B.gaussian_distribution(x, 0.5, 1.2, 0.7 + 0.2 * a) for x in xs[a]]) for a in range(3)]
+    psi0 = g1[0][:, None, None] * g1[1][None, :, None] * g1[2][None, None, :]
+    wf = B.WaveFunction(psi0, grids) if dtype is None else B.WaveFunction(psi0, grids, dtype=dtype)
+    wf.normalize(1.0)
+    p = B.Propagation()
+    p.set_wave_function(wf)
+    p.set_time_grid(dt, steps, im_time)
+    X, Y, Z = np.meshgrid(*xs, indexing="ij")
+    V = 0.5 * (X * X + 1.3 * Y * Y + 0.8 * Z * Z)
+    p.add_propagator(B.n_dim_propagator(B.exp_propagator(V, B.select_step(False, dt, im_time))))
+    for a in range(3):
+        p.add_transformation(B.fft_transformation(grids[a], "k" + "xyz"[a]))
+    for a in range(3):
+        p.add_propagator(B.one_dim_propagator(B.exp_propagator(B.kinetic_hamiltonian(grids[a], 1.0),
+                                                               B.select_step(True, dt, im_time)), a))
+    return p
+
+
+def two_tables_1d(B, n=1024, steps=200, dt=0.01, dtype=None, im_time=False):
+    """1-D stack with two distinct full-grid tables next to the FFT pass (two register prefetches) and a
+    LeakControl with a loss checker (norms and a scalar program in the FFT passes)."""
+    g = B.Grid.new_linear_continuos("x", -20.0, 20.0, n, 0)
+    x = g.nodes
+    arr = np.array([B.gaussian_distribution(xi, 1.5, 1.0, 0.5) for xi in x])
+    wf = B.WaveFunction(arr, [g]) if dtype is None else B.WaveFunction(arr, [g], dtype=dtype)
+    wf.normalize(1.0)
+    p = B.Propagation()
+    p.set_wave_function(wf)
+    p.set_time_grid(dt, steps, im_time)
+    lc = B.leak_control()
+    lc.add_loss_checker("leak")
+    p.add_control(lc, B.APPLY_FIRST | B.APPLY_SECOND)
+    V1 = 0.5 * x * x
+    V2 = 0.05 * np.cos(x)
+    p.add_propagator(B.n_dim_propagator(B.exp_propagator(V1, B.select_step(False, dt, im_time))))
+    p.add_propagator(B.n_dim_propagator(B.exp_propagator(V2, B.select_step(False, dt, im_time))))
+    p.add_transformation(B.fft_transformation(g, "k"))
+    p.add_propagator(B.one_dim_propagator(B.exp_propagator(B.kinetic_hamiltonian(g, 1.0),
+                                                           B.select_step(True, dt, im_time)), 0))
+    return p
+
+
+def run_both(make_so, make_oracle):
+    out = {}
+    for kern in ("lean", "general"):
+        p = make_so()
+        p.set_fft_kernel(kern)
+        p.propagate()
+        out[kern] = p.wave_function().array
+    po = make_oracle()
+    po.propagate()
+    return out["lean"], out["general"], po.wave_function().array
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32", "f32x"])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (6, 64, 6), (6, 6, 64), (4, 8, 256)])
+def test_lean_and_general_fft_3d(dtype, shape):
+    """Strided lanes with partial tiles (6, 64, 6), contiguous lanes with a partial last tile (6, 6, 64),
+    small tiles below the workgroup's element count (4, 8, 256 along x), cubes."""
+    lean, gen, ref = run_both(lambda: box_3d(so, shape, dtype=dtype), lambda: box_3d(O, shape))
+    assert rel_l2(lean, ref) < TOL_ORACLE[dtype], rel_l2(lean, ref)
+    assert rel_l2(gen, ref) < TOL_ORACLE[dtype], rel_l2(gen, ref)
+    assert rel_l2(lean, gen) < TOL_PAIR[dtype], rel_l2(lean, gen)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32", "f32x"])
+@pytest.mark.parametrize("im_time", [False, True])
+def test_lean_and_general_fft_two_tables_leak(dtype, im_time):
+    """Single-lane tiles (workgroup-wide radix passes), two prefetched full-grid tables, LeakControl norms and
+    its scalar program; imaginary time renormalises every step."""
+    lean, gen, ref = run_both(lambda: two_tables_1d(so, dtype=dtype, im_time=im_time),
+                              lambda: two_tables_1d(O, im_time=im_time))
+    assert rel_l2(lean, ref) < TOL_ORACLE[dtype], rel_l2(lean, ref)
+    assert rel_l2(gen, ref) < TOL_ORACLE[dtype], rel_l2(gen, ref)
+    assert rel_l2(lean, gen) < TOL_PAIR[dtype] * 10, rel_l2(lean, gen)
+
+
+def test_lean_and_general_fft_radial_angular_leak():
+    """The radial x angular stack (radial FFT of 256 points, border damping, loss checker, LeakControl) in FP64."""
+    kw = dict(n_r=256, n_theta=32, steps=60, dt=0.002, leak=True)
+    lean, gen, ref = run_both(lambda: W.radial_angular(so, dtype="f64", **kw)[0],
+                              lambda: W.radial_angular(O, **kw)[0])
+    assert rel_l2(lean, ref) < 1e-10
+    assert rel_l2(gen, ref) < 1e-10
+    assert rel_l2(lean, gen) < 1e-12
+
+
+def test_lean_and_general_fft_harmonic_fixture(fixtures):
+    """tests/harmonic_trap.rs:34-111 (imaginary time, StateSaver, LeakControl) on both kernels vs the crate's frames."""
+    so.set_default_dtype("f64")
+    for kern in ("lean", "general"):
+        p, sv, _ = S.harmonic_trap(so, True, "harmonic_imaginary")
+        p.set_fft_kernel(kern)
+        p.propagate()
+        data, _ = sv.frames(50)
+        cols = fixtures["harmonic_imaginary__cols"]
+        ref = fixtures["harmonic_imaginary__frames"]
+        assert np.max(np.abs(data[:, cols] - ref)) / np.max(np.abs(ref)) < 1e-11
+
+
+def test_fft_kernel_choice_rejects_unknown():
+    p, _ = W.harmonic_1d(so, dtype="f64", steps=2)
+    with pytest.raises(Exception):
+        p.set_fft_kernel("fastest")
