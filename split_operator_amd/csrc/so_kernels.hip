@@ -386,6 +386,103 @@ __device__ __forceinline__ void radix_pass_p2(T2* lds, int tl, int n, int ld, in
     __syncthreads();
 }
 
+// radix_pass_p2 with the lane length N and the stride LS known at compile
+// time: every shift, mask and padded offset is a constant (LDS offsets fold
+// into the instructions' immediate fields).
+template <int R, int EPT, int N, int LS, typename T, typename T2>
+__device__ __forceinline__ void radix_pass_nc(T2* lds, int tl, int ld, const T2* tw, T scale, T scale_lo, T sg,
+                                              bool last, int tid, bool wl) {
+  constexpr int MAXB = EPT / R > 0 ? EPT / R : 1;
+  constexpr int NB = N / R;
+  constexpr int NB_SH = __builtin_ctz(NB);
+  constexpr int LS_SH = __builtin_ctz(LS);
+  const int lw = wl ? tl / (NTHREADS / 64) : tl;
+  const int l0 = wl ? (tid >> 6) * lw : 0;
+  const int total = lw * NB;
+  const int bstride = wl ? 64 : NTHREADS;
+  if (wl) tid &= 63;
+  T2 v[MAXB][R];
+  int dpos[MAXB];
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int b = min(tid + q * bstride, total - 1);
+    const int l = (b >> NB_SH) + l0;
+    const int j = b & (NB - 1);
+    const int k = j & (LS - 1);
+    const int jd = j >> LS_SH;
+    const T2* src = lds + l * ld + fpad(j);
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[q][r] = lld(&src[r * NB + ((r * NB) >> 5)]);
+    if constexpr (LS > 1) {
+      const T2* twk = tw + k;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        T2 w = lld(&twk[(r - 1) * LS]);
+        w.y *= sg;
+        v[q][r] = cmul(v[q][r], w);
+      }
+    }
+    dft<R>(v[q], sg);
+    dpos[q] = l * ld + fpad((jd << LS_SH) * R + k);
+  }
+  if (wl)
+    wave_lds_sync();
+  else
+    __syncthreads();
+  const bool exact = scale_lo == (T)0;
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int b = tid + q * bstride;
+    if (b < total) {
+      T2* dst = lds + dpos[q];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        T2 z = v[q][r];
+        if (last) {
+          if (exact) {
+            z.x *= scale;
+            z.y *= scale;
+          } else {
+            z.x = fma(z.x, scale, z.x * scale_lo);
+            z.y = fma(z.y, scale, z.y * scale_lo);
+          }
+        }
+        lst(&dst[r * LS + ((r * LS) >> 5)], z);
+      }
+    }
+  }
+  if (wl && !last)
+    wave_lds_sync();
+  else
+    __syncthreads();
+}
+
+// The FFT of lanes of a compile-time power-of-two length N with the host's
+// plan (radix 8 passes, then one radix 4 or 2): pass p has stride Ls_p (the
+// product of the earlier radices) and its twiddles start at Ls_p - 1.
+template <int EPT, int N, int LS, typename T, typename T2>
+__device__ __forceinline__ void fft_passes_nc(T2* lds, int tl, int ld, const T2* tw, T scale, T scale_lo, T sg,
+                                              int tid, bool wl) {
+  constexpr int REM = N / LS;
+  if constexpr (REM >= 8) {
+    radix_pass_nc<8, EPT, N, LS>(lds, tl, ld, tw + (LS - 1), scale, scale_lo, sg, REM == 8, tid, wl);
+    if constexpr (REM > 8) fft_passes_nc<EPT, N, LS * 8>(lds, tl, ld, tw, scale, scale_lo, sg, tid, wl);
+  } else if constexpr (REM == 4) {
+    radix_pass_nc<4, EPT, N, LS>(lds, tl, ld, tw + (LS - 1), scale, scale_lo, sg, true, tid, wl);
+  } else if constexpr (REM == 2) {
+    radix_pass_nc<2, EPT, N, LS>(lds, tl, ld, tw + (LS - 1), scale, scale_lo, sg, true, tid, wl);
+  }
+}
+
+template <int EPT, int N, typename T, typename T2>
+__device__ __forceinline__ void fft_lds_n(T2* lds, int tl, int ld, const T2* tw, double scale_d, bool inverse, int tid,
+                                          bool wl) {
+  const T scale = (T)scale_d;
+  const T scale_lo = (T)(scale_d - (double)scale);
+  const T sg = inverse ? (T)-1 : (T)1;
+  fft_passes_nc<EPT, N, 1>(lds, tl, ld, tw, scale, scale_lo, sg, tid, wl);
+}
+
 template <int EPT, typename T, bool P2 = false, typename T2>
 __device__ __forceinline__ void fft_lds(T2* lds, int tl, int n, int ld, const T2* tw, double scale_d, bool inverse,
                                         int nrad, const int32_t* rad, int tid, bool wl = false) {
@@ -1372,7 +1469,7 @@ __global__ __launch_bounds__(NTHREADS) void xform_pass_kernel(const PassDesc* __
 // Same stages, transforms and reductions as xform_pass_body.
 constexpr int FCH = 4;
 
-template <typename S, typename T, int EPT, int PFN>
+template <typename S, typename T, int EPT, int PFN, int N>
 __device__ __forceinline__ void fft_lean_body(const PassDesc* __restrict__ d, int64_t tile, int64_t ntiles,
                                               bool inherit) {
   using T2 = typename CT<T>::T2;
@@ -1382,7 +1479,7 @@ __device__ __forceinline__ void fft_lean_body(const PassDesc* __restrict__ d, in
   __shared__ LaneTab<T> lt;
   const int tid = threadIdx.x;
   const int tl = d->tl;
-  const int n = (int)d->n;
+  const int n = N > 0 ? N : (int)d->n;
   const int ld = d->ld;
   const int64_t inner = d->inner;
   const int tot = tl * n;
@@ -1519,14 +1616,19 @@ __device__ __forceinline__ void fft_lean_body(const PassDesc* __restrict__ d, in
       stage_chunks(d->st[1], 1, 0);
       __syncthreads();
     }
-    fft_lds<EPT, T, true>(cur, tl, n, ld, twl, x.scale, x.inverse != 0, x.nrad, x.rad, tid, tl >= NTHREADS / 64);
+    if constexpr (N > 0)
+      fft_lds_n<EPT, N, T>(cur, tl, ld, twl, x.scale, x.inverse != 0, tid, tl >= NTHREADS / 64);
+    else
+      fft_lds<EPT, T, true>(cur, tl, n, ld, twl, x.scale, x.inverse != 0, x.nrad, x.rad, tid, tl >= NTHREADS / 64);
   }
   stage_chunks(d->st[nx], 1, 1);
   finish_pass(d, acc, tid, tile, ntiles);
 }
 
-// FP32 tiles of 8 elements per thread are held to 168 VGPRs (3 waves per SIMD).
-template <typename S, typename T, int EPT, int PFN>
+// N > 0: lanes of exactly N points (the FFT plan unrolled at compile time);
+// N = 0: any power-of-two length. FP32 tiles of 8 elements per thread are held
+// to 168 VGPRs (3 waves per SIMD).
+template <typename S, typename T, int EPT, int PFN, int N = 0>
 __global__ __launch_bounds__(NTHREADS)
 __attribute__((amdgpu_waves_per_eu((sizeof(T) == 4 && EPT == 8 && PFN <= 1) ? 3 : 1))) void fft_lean_kernel(const PassDesc* __restrict__ d, int64_t ntiles,
                                                             int inherit_block) {
@@ -1535,7 +1637,7 @@ __attribute__((amdgpu_waves_per_eu((sizeof(T) == 4 && EPT == 8 && PFN <= 1) ? 3 
     run_inherited(d, threadIdx.x);
     return;
   }
-  fft_lean_body<S, T, EPT, PFN>(d, b, ntiles, b == 0 && !inherit_block);
+  fft_lean_body<S, T, EPT, PFN, N>(d, b, ntiles, b == 0 && !inherit_block);
 }
 
 // One launch for the same pass position of many independent wave functions
@@ -1867,11 +1969,60 @@ static void set_lds_attr_lean_all() {
   set_lds_attr_lean<S, T, 8, 1>();
   set_lds_attr_lean<S, T, 8, 2>();
   set_lds_attr_lean<S, T, 16, 0>();
+  if constexpr (sizeof(S) == 4 && sizeof(T) == 4) {
+    (void)hipFuncSetAttribute((const void*)fft_lean_kernel<S, T, 8, 0, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)SOD_MAX_DYN_LDS);
+    (void)hipFuncSetAttribute((const void*)fft_lean_kernel<S, T, 8, 1, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)SOD_MAX_DYN_LDS);
+    (void)hipFuncSetAttribute((const void*)fft_lean_kernel<S, T, 8, 0, 2048>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)SOD_MAX_DYN_LDS);
+    (void)hipFuncSetAttribute((const void*)fft_lean_kernel<S, T, 8, 1, 2048>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)SOD_MAX_DYN_LDS);
+  }
+}
+
+// The host's plan for a power-of-two n: radix 8 passes, then one radix 4 or 2.
+static bool plan_is_standard(const XformDesc& x, int64_t n) {
+  int p = 0;
+  while ((int64_t(1) << p) < n) ++p;
+  int i = 0;
+  for (; p >= 3; p -= 3, ++i)
+    if (i >= x.nrad || x.rad[i] != 8) return false;
+  if (p == 2 && (i >= x.nrad || x.rad[i++] != 4)) return false;
+  if (p == 1 && (i >= x.nrad || x.rad[i++] != 2)) return false;
+  return i == x.nrad;
+}
+
+// FP32 passes over lanes of 256 or 2048 points with 8 elements per thread run
+// on kernels with the FFT plan unrolled for that length.
+template <typename S, typename T>
+static bool launch_fft_lean_n(const PassDesc* d_desc, const PassDesc& h, dim3 grid, size_t lds, int64_t nt, int ib,
+                              hipStream_t s) {
+  if constexpr (!(sizeof(S) == 4 && sizeof(T) == 4)) {
+    return false;
+  } else {
+    if (h.ept != 8 || h.n_pf > 1 || (h.n != 256 && h.n != 2048)) return false;
+    for (int i = 0; i < h.nx; ++i)
+      if (!plan_is_standard(h.xf[i], h.n)) return false;
+    if (h.n == 256) {
+      if (h.n_pf == 0)
+        fft_lean_kernel<S, T, 8, 0, 256><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib);
+      else
+        fft_lean_kernel<S, T, 8, 1, 256><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib);
+    } else {
+      if (h.n_pf == 0)
+        fft_lean_kernel<S, T, 8, 0, 2048><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib);
+      else
+        fft_lean_kernel<S, T, 8, 1, 2048><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib);
+    }
+    return true;
+  }
 }
 
 template <typename S, typename T>
 static void launch_fft_lean(const PassDesc* d_desc, const PassDesc& h, dim3 grid, size_t lds, int64_t nt, int ib,
                             hipStream_t s) {
+  if (launch_fft_lean_n<S, T>(d_desc, h, grid, lds, nt, ib, s)) return;
   const int key = h.ept * 4 + h.n_pf;
   switch (key) {
     case 16: fft_lean_kernel<S, T, 4, 0><<<grid, NTHREADS, lds, s>>>(d_desc, nt, ib); break;
