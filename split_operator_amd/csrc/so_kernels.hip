@@ -1367,23 +1367,32 @@ __device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, 
   T2* cur = (T2*)smem;
   T2* oth = cur + tl * ld;
   const T2* twl = (const T2*)(smem + d->lds_arr_off[0]);
-  BRegs<T> br;
-  if constexpr (KIND == XF_MAT) {
-    if (mat_fast<T>(d->xf[0], tl)) mat_prefetch<T>(d->xf[0], tl, tid, br);
-  }
-  setup_lanes<T>(d, lt, tile, tid);
-  __syncthreads();
   S2* psi = (S2*)d->psi;
   const bool strided = inner > 1;
   int ql[EPT], qk[EPT];
   map_global<EPT>(ql, qk, tid, tot, tl, n, strided);
+  // the tile's origin and valid lanes (what setup_lanes records per lane), so
+  // the psi and table loads go out first, ahead of the lane table, the GEMM
+  // operand prefetch and the LDS staging
+  int64_t base0;
+  int nvalid;
+  if (strided) {
+    const int64_t o = tile / d->tiles_per_outer;
+    const int64_t i0 = (tile - o * d->tiles_per_outer) * tl;
+    base0 = o * (int64_t)n * inner + i0;
+    nvalid = (int)min((int64_t)tl, inner - i0);
+  } else {
+    base0 = tile * tl * (int64_t)n;
+    nvalid = (int)min((int64_t)tl, d->outer - tile * tl);
+  }
+  const int64_t lstride = strided ? 1 : n;
+  auto gidx = [&](int q) { return base0 + ql[q] * lstride + (int64_t)qk[q] * inner; };
   T2 z[EPT];
 #pragma unroll
   for (int q = 0; q < EPT; ++q) {
     z[q].x = 0;
     z[q].y = 0;
-    if (tid + q * NTHREADS < tot && lt.valid[ql[q]])
-      z[q] = cvt2<T>(gld(&psi[lt.base[ql[q]] + (int64_t)qk[q] * inner]));
+    if (tid + q * NTHREADS < tot && ql[q] < nvalid) z[q] = cvt2<T>(gld(&psi[gidx(q)]));
   }
   // full-grid tables of the later stages, into registers
   T2 pf[PFN > 0 ? PFN : 1][EPT];
@@ -1394,13 +1403,18 @@ __device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, 
         const T2* tab = (const T2*)d->pf_src[u];
 #pragma unroll
         for (int q = 0; q < EPT; ++q)
-          if (tid + q * NTHREADS < tot) pf[u][q] = gld(&tab[lt.base[ql[q]] + (int64_t)qk[q] * inner]);
+          if (tid + q * NTHREADS < tot && ql[q] < nvalid) pf[u][q] = gld(&tab[gidx(q)]);
       }
   }
+  BRegs<T> br;
+  if constexpr (KIND == XF_MAT) {
+    if (mat_fast<T>(d->xf[0], tl)) mat_prefetch<T>(d->xf[0], tl, tid, br);
+  }
+  setup_lanes<T>(d, lt, tile, tid);
   // stage small arrays (twiddles, 1-D tables, weights) into LDS while the psi
   // loads above are in flight
   stage_lds_arrays(d, smem, tid);
-  __syncthreads();
+  __syncthreads();  // lane table and LDS arrays ready
   if (inherit) run_inherited(d, tid);
   Acc acc;
   stage_vec<T, EPT, PFN>(d->st[0], z, ql, qk, tid, tot, lt, d, acc, smem, pf);
@@ -1441,8 +1455,7 @@ __device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, 
   stage_vec<T, EPT, PFN>(d->st[nx], z, ql, qk, tid, tot, lt, d, acc, smem, pf);
 #pragma unroll
   for (int q = 0; q < EPT; ++q)
-    if (tid + q * NTHREADS < tot && lt.valid[ql[q]])
-      gst(&psi[lt.base[ql[q]] + (int64_t)qk[q] * inner], cvt2<S>(z[q]));
+    if (tid + q * NTHREADS < tot && ql[q] < nvalid) gst(&psi[gidx(q)], cvt2<S>(z[q]));
   finish_pass(d, acc, tid, tile, ntiles);
 }
 
