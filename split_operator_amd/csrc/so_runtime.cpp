@@ -1203,7 +1203,14 @@ std::unique_ptr<CompiledPass> Propagation::build(const PassB& b, bool immediate,
           sod::ElemOp& o = h.st[st].op[i];
           if (o.kind == sod::EK_MUL1D && o.axis == axis) o.lds = add_arr(o.table, n * (int64_t)cs);
           if (o.kind == sod::EK_MUL1D && o.axis != axis && o.lf < 0) o.lf = add_lf({{o.table, o.axis}});
-          if (o.kind == sod::EK_SCALE) o.lf = add_hs(o.slot);
+          if (o.kind == sod::EK_SCALE) {
+            // the factor is read at kernel start: no program that runs inside
+            // this launch (the inherited one) may write it
+            for (int i2 = 0; i2 < h.prev_nscalar; ++i2)
+              if (h.prev_sc[i2].a == o.slot)
+                fail(SO_ERR_STATE, "scale slot %d is written by the program this pass inherits", o.slot);
+            o.lf = add_hs(o.slot);
+          }
           if (o.kind == sod::EK_NORM) o.lds = add_arr(o.w_axis, n * 8);
           if (o.kind == sod::EK_MULND) o.pf = add_pf(o.table);
         }
