@@ -1402,8 +1402,13 @@ __device__ __forceinline__ void xform_pass_body(const PassDesc* __restrict__ d, 
       if (u < d->n_pf) {
         const T2* tab = (const T2*)d->pf_src[u];
 #pragma unroll
-        for (int q = 0; q < EPT; ++q)
+        for (int q = 0; q < EPT; ++q) {
+          // elements of lanes past the tile's end multiply zeros: they hold
+          // zeros too (uninitialised registers could hold NaN bit patterns)
+          pf[u][q].x = 0;
+          pf[u][q].y = 0;
           if (tid + q * NTHREADS < tot && ql[q] < nvalid) pf[u][q] = gld(&tab[gidx(q)]);
+        }
       }
   }
   BRegs<T> br;

@@ -33,6 +33,29 @@ def _instances(B, count, dtype, steps=60, dt=0.05, seed0=4100):
     return out
 
 
+@pytest.mark.parametrize("seed0", [4120, 4130, 4140])
+def test_batch_partial_tiles_with_tables_stay_finite(seed0):
+    """FP32 batches under hipGraph replay whose members have partial tiles (lanes past the tile's end) in passes
+    that prefetch full-grid tables and measure norms: those lanes must hold zeros, not whatever the registers held
+    (a NaN there reaches the norm, the renormalisation factor and every element). Equal to the members run one by
+    one."""
+    steps = 40
+    batch = so.Batch()
+    batch.set_graphs(True)
+    mem = _instances(so, 6, "f32", steps, seed0=seed0)
+    for p in mem:
+        batch.add(p)
+    batch.steps(steps)
+    batch.flush()
+    solo = _instances(so, 6, "f32", steps, seed0=seed0)
+    for p in solo:
+        p.steps(steps)
+    for a, b in zip(mem, solo):
+        x = a.wave_function().array
+        assert np.isfinite(x).all()
+        assert rel_l2(x, b.wave_function().array) < 2e-6
+
+
 @pytest.mark.parametrize("dtype,tol", [("f64", 1e-13), ("f32", 2e-6)])
 def test_batch_matches_individual(dtype, tol):
     steps = 40
