@@ -482,6 +482,13 @@ struct Propagation {
   // scalar kernel) runs its scalar program in workgroup 0.
   CompiledPass* pending_prog = nullptr;
   std::string last_closed_key;
+  // Renormalisation factors (EK_SCALE) computed by the open pass's own scalar
+  // program. Instead of closing the pass where the factor is applied, the
+  // factor is carried to the next pass boundary: a scale commutes with every
+  // diagonal multiply and lane transform that follows. It is applied earlier,
+  // where something observes the magnitude of psi (a norm, a saver, a
+  // lane-matrix pass, a flush) or a scalar op rewrites its slot.
+  std::vector<int> pending_scale;
 
   // ----------------------------------------------------------- action emission
   // `immediate`: the next op reads a slot this pass's program writes, so the
@@ -489,17 +496,44 @@ struct Propagation {
   void close_open(bool immediate = false) {
     if (open.empty()) {
       open = PassB();
+      place_pending();
       return;
     }
     // a pass that repeats back to back (one pass per step) finishes its own
     // bookkeeping: deferring it would chain every launch to the previous one
     std::string ck = pass_key(open);
     if (ck == last_closed_key) immediate = true;
+    // the next pass applies the carried factors at its start
+    for (int sl : pending_scale)
+      if (open.writes(sl)) immediate = true;
     last_closed_key.swap(ck);
     Launch L;
     L.pass = compile(open, immediate);
     emitted.push_back(L);
     open = PassB();
+    place_pending();
+  }
+  // the carried factors open the (new, empty) pass
+  void place_pending() {
+    for (int sl : pending_scale) {
+      ElemA e;
+      e.kind = sod::EK_SCALE;
+      e.slot = sl;
+      open.st[open.stage].push_back(e);
+    }
+    pending_scale.clear();
+  }
+  // applies the carried factors now: in this pass when an earlier pass
+  // computed them, else at the start of a new pass
+  void resolve_pending() {
+    if (pending_scale.empty()) return;
+    bool own = false;
+    for (int sl : pending_scale) own = own || open.writes(sl);
+    if (own || (int)(open.st[open.stage].size() + pending_scale.size()) > sod::MAX_ELEM) {
+      close_open(true);
+      return;
+    }
+    place_pending();
   }
   // launches a scalar kernel for a still-deferred program
   void finish_pending() {
@@ -509,12 +543,16 @@ struct Propagation {
     launch_pass(cp);
   }
   void emit_elem(const ElemA& e) {
-    if (e.kind == sod::EK_SCALE && open.writes(e.slot)) close_open(true);
+    if (e.kind == sod::EK_SCALE && open.writes(e.slot)) {
+      pending_scale.push_back(e.slot);
+      return;
+    }
     if ((int)open.st[open.stage].size() >= sod::MAX_ELEM) close_open();
     open.st[open.stage].push_back(e);
   }
   void emit_norm() {
     // a norm of the current state with the current grids, into slot s_last
+    resolve_pending();
     if (open.nacc >= sod::MAX_ACC || (int)open.st[open.stage].size() >= sod::MAX_ELEM ||
         (int)open.sc.size() >= sod::MAX_SCALAR)
       close_open();
@@ -533,6 +571,7 @@ struct Propagation {
     open.nacc++;
   }
   void emit_scalar(const sod::ScalarOp& s) {
+    if (std::find(pending_scale.begin(), pending_scale.end(), s.a) != pending_scale.end()) resolve_pending();
     if ((int)open.sc.size() >= sod::MAX_SCALAR) close_open();
     open.sc.push_back(s);
     open.written.push_back(s.a);
@@ -557,6 +596,7 @@ struct Propagation {
     open.stage = 1;
   }
   void emit_lanemat(so_op* op, int which) {
+    resolve_pending();
     close_open();
     Launch L;
     L.lanemat = op;
@@ -564,6 +604,7 @@ struct Propagation {
     emitted.push_back(L);
   }
   void emit_save(so_op* sv, int64_t col) {
+    resolve_pending();
     close_open();
     Launch L;
     L.is_save = true;
@@ -1417,6 +1458,7 @@ void Propagation::launch_pass(CompiledPass* cp) {
 }
 
 void Propagation::flush() {
+  resolve_pending();
   if (!open.empty()) close_open();
   for (auto& L : emitted) launch(L);
   emitted.clear();
@@ -1426,6 +1468,8 @@ void Propagation::flush() {
 std::string Propagation::state_sig() const {
   std::string k = pass_key(open);
   key_add(k, &pending_prog, sizeof(void*));
+  for (int sl : pending_scale) key_add(k, &sl, sizeof(sl));
+  k.push_back(0);
   k += last_closed_key;
   k.push_back(wf->possible_norm_change ? 1 : 0);
   for (int a = 0; a < wf->ndim; ++a) {
@@ -2479,6 +2523,7 @@ static void prop_reset_compiled(Propagation& p) {
   }
   p.cache.clear();
   p.open = PassB();
+  p.pending_scale.clear();
   p.emitted.clear();
   p.pending_prog = nullptr;
   p.last_closed_key.clear();
