@@ -1535,13 +1535,37 @@ __device__ __forceinline__ void fft_lean_body(const PassDesc* __restrict__ d, in
   // full tile: every element of every thread is a valid lane element
   const bool full = nvalid == tl && tot == EPT * NTHREADS;
   auto live = [&](int f) { return f < tot && lane_of(f) < nvalid; };
+  // Unrolled lengths, full tiles of 8 elements per thread: thread tid loads the
+  // elements of butterfly tid of the first radix-8 pass (lane blane, indices
+  // bj + r N/8), applies the first stage and that butterfly in registers and
+  // writes its outputs to LDS, skipping one LDS round trip and barrier.
+  constexpr int NB = N >= 64 ? N / 8 : 1;
+  const bool direct = N >= 64 && EPT == 8 && full;
+  const int blane = strided ? (tid & (tl - 1)) : (tid >> __builtin_ctz((unsigned)NB));
+  const int bj = strided ? (tid >> tsh) : (tid & (NB - 1));
+  auto boff = [&](int r) {
+    const uint32_t k = (uint32_t)(bj + r * NB);
+    return strided ? (uint32_t)blane + k * inner32 : (uint32_t)blane * (uint32_t)n + k;
+  };
   T2* cur = (T2*)smem;
   const T2* twl = (const T2*)(smem + d->lds_arr_off[0]);
   // every global read of the tile is issued first
   const S2* psi_in = (const S2*)d->psi + base0;
   T2 z[EPT];
   T2 pf[PF1][EPT];
-  if (full) {
+  if (direct) {
+#pragma unroll
+    for (int r = 0; r < EPT; ++r)
+      z[r] = cvt2<T>(gld((const S2*)((const char*)psi_in + boff(r) * (uint32_t)sizeof(S2))));
+    if constexpr (PFN > 0) {
+#pragma unroll
+      for (int u = 0; u < PFN; ++u) {
+        const T2* tab = (const T2*)d->pf_src[u] + base0;
+#pragma unroll
+        for (int r = 0; r < EPT; ++r) pf[u][r] = gld((const T2*)((const char*)tab + boff(r) * (uint32_t)sizeof(T2)));
+      }
+    }
+  } else if (full) {
 #pragma unroll
     for (int q = 0; q < EPT; ++q)
       z[q] = cvt2<T>(gld((const S2*)at(psi_in, tid + q * NTHREADS, sizeof(S2))));
@@ -1625,15 +1649,46 @@ __device__ __forceinline__ void fft_lean_body(const PassDesc* __restrict__ d, in
       }
     }
   };
-  stage_chunks(d->st[0], 0, 0);
-  __syncthreads();
   const int nx = d->nx;
+  if constexpr (N >= 64 && EPT == 8) {
+    if (direct) {
+      int ql[EPT], qk[EPT];
+#pragma unroll
+      for (int r = 0; r < EPT; ++r) {
+        ql[r] = blane;
+        qk[r] = bj + r * NB;
+      }
+      if (d->st[0].n > 0) stage_vec<T, EPT, PFN>(d->st[0], z, ql, qk, tid, tot, lt, d, acc, smem, pf);
+      const XformDesc& x0 = d->xf[0];
+      const T sg = x0.inverse ? (T)-1 : (T)1;
+      dft<8>(z, sg);
+      // pass stride 1: outputs bj*8 + r, one pad group holds all eight
+      T2* dst = cur + blane * ld + fpad(bj * 8);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) lst(&dst[r], z[r]);
+      // contiguous tiles: wave w wrote lanes [w tl/4, (w+1) tl/4), the lanes
+      // it owns in the wave-local passes that follow
+      const bool wl = !strided && tl >= NTHREADS / 64;
+      if (wl)
+        wave_lds_sync();
+      else
+        __syncthreads();
+      const T scale = (T)x0.scale;
+      const T scale_lo = (T)(x0.scale - (double)scale);
+      fft_passes_nc<EPT, N, 8>(cur, tl, ld, twl, scale, scale_lo, sg, tid, tl >= NTHREADS / 64);
+    }
+  }
+  if (!direct) {
+    stage_chunks(d->st[0], 0, 0);
+    __syncthreads();
+  }
   for (int xi = 0; xi < nx; ++xi) {
     const XformDesc& x = d->xf[xi];
     if (xi == 1 && d->st[1].n > 0) {
       stage_chunks(d->st[1], 1, 0);
       __syncthreads();
     }
+    if (direct && xi == 0) continue;
     if constexpr (N > 0)
       fft_lds_n<EPT, N, T>(cur, tl, ld, twl, x.scale, x.inverse != 0, tid, tl >= NTHREADS / 64);
     else
