@@ -2042,7 +2042,7 @@ static void set_lds_attr_lean_all() {
   set_lds_attr_lean<S, T, 8, 1>();
   set_lds_attr_lean<S, T, 8, 2>();
   set_lds_attr_lean<S, T, 16, 0>();
-  if constexpr (sizeof(S) == 4 && sizeof(T) == 4) {
+  if constexpr (sizeof(S) == 4) {
     (void)hipFuncSetAttribute((const void*)fft_lean_kernel<S, T, 8, 0, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)SOD_MAX_DYN_LDS);
     (void)hipFuncSetAttribute((const void*)fft_lean_kernel<S, T, 8, 1, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2066,12 +2066,13 @@ static bool plan_is_standard(const XformDesc& x, int64_t n) {
   return i == x.nrad;
 }
 
-// FP32 passes over lanes of 256 or 2048 points with 8 elements per thread run
-// on kernels with the FFT plan unrolled for that length.
+// Passes over lanes of 256 or 2048 points with 8 elements per thread and FP32
+// storage (FP32 or FP64 arithmetic) run on kernels with the FFT plan unrolled
+// for that length.
 template <typename S, typename T>
 static bool launch_fft_lean_n(const PassDesc* d_desc, const PassDesc& h, dim3 grid, size_t lds, int64_t nt, int ib,
                               hipStream_t s) {
-  if constexpr (!(sizeof(S) == 4 && sizeof(T) == 4)) {
+  if constexpr (sizeof(S) != 4) {
     return false;
   } else {
     if (h.ept != 8 || h.n_pf > 1 || (h.n != 256 && h.n != 2048)) return false;

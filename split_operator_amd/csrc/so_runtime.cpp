@@ -1101,12 +1101,13 @@ std::unique_ptr<CompiledPass> Propagation::build(const PassB& b, bool immediate,
     }
     h.tl = tl;
     // FFT-only passes: the lean kernel when every wave of the workgroup owns
-    // whole lanes (throughput-bound tiles), or for FP32 lanes of 256 or 2048
-    // points, whose lean kernels have the FFT plan unrolled; other single-lane
-    // tiles of long lanes are latency-bound and keep the general kernel
+    // whole lanes (throughput-bound tiles), or for FP32-stored lanes of 256 or
+    // 2048 points, whose lean kernels have the FFT plan unrolled; other
+    // single-lane tiles of long lanes are latency-bound and keep the general
+    // kernel
     h.fft_kernel = 0;
     if (b.nx > 0 && !has_mat) {
-      const bool unrolled = arith_dt(dt) == 0 && store_dt(dt) == 0 && (n == 256 || n == 2048);
+      const bool unrolled = store_dt(dt) == 0 && (n == 256 || n == 2048);
       h.fft_kernel = force_kernel >= 0 ? force_kernel : ((tl >= 4 || unrolled) ? 1 : 0);
     }
     if (b.nx == 0) {

@@ -102,9 +102,9 @@ def run_both(make_so, make_oracle):
 @pytest.mark.parametrize("shape", [(32, 32, 32), (6, 64, 6), (6, 6, 64), (4, 8, 256), (256, 8, 8)])
 def test_lean_and_general_fft_3d(dtype, shape):
     """Strided lanes with partial tiles (6, 64, 6), contiguous lanes with a partial last tile (6, 6, 64),
-    small tiles below the workgroup's element count (4, 8, 256 along x), cubes; FP32 lanes of 256 points, contiguous
-    (4, 8, 256) and strided with the potential prefetched (256, 8, 8), run on the kernels specialised for that
-    length."""
+    small tiles below the workgroup's element count (4, 8, 256 along x), cubes; FP32-stored lanes of 256 points,
+    contiguous (4, 8, 256) and strided with the potential prefetched (256, 8, 8), run on the kernels specialised for
+    that length."""
     lean, gen, ref = run_both(lambda: box_3d(so, shape, dtype=dtype), lambda: box_3d(O, shape))
     assert rel_l2(lean, ref) < TOL_ORACLE[dtype], rel_l2(lean, ref)
     assert rel_l2(gen, ref) < TOL_ORACLE[dtype], rel_l2(gen, ref)
@@ -133,10 +133,10 @@ def test_lean_and_general_fft_radial_angular_leak():
     assert rel_l2(lean, gen) < 1e-12
 
 
-@pytest.mark.parametrize("dtype", ["f32", "f64"])
+@pytest.mark.parametrize("dtype", ["f32", "f32x", "f64"])
 def test_lean_and_general_fft_radial_2048(dtype):
-    """Radial lanes of 2048 points (single-lane tiles; in FP32 the kernel specialised for 2048 points) with the
-    Rot/2 table prefetched, border damping and the loss checker."""
+    """Radial lanes of 2048 points (single-lane tiles; with FP32 storage the kernel specialised for 2048 points)
+    with the Rot/2 table prefetched, border damping and the loss checker."""
     kw = dict(n_r=2048, n_theta=32, steps=40, dt=0.002, leak=True)
     lean, gen, ref = run_both(lambda: W.radial_angular(so, dtype=dtype, **kw)[0],
                               lambda: W.radial_angular(O, **kw)[0])
