@@ -257,8 +257,8 @@ def learned_warm_start(so, args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md and not launched", file=sys.stderr)
+        print(f"bench.py: this benchmark runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding of the path is not launched", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
