@@ -76,6 +76,9 @@ struct XformDesc {
   int32_t bf16x3;          // MAT, FP32 arithmetic: product as split-BF16 x3 on BF16 MFMA
   const void* Bh;          // bf16x3: BF16 high parts of B, [32][npad][4] (element (k, c) at ((k/4) npad + c) 4 + k%4)
   const void* Bl;          // bf16x3: BF16 low parts (B - high), same layout
+  const void* Bf;          // MAT, kpad <= 128: B in MFMA-fragment order, [npad/16][128/G][64][G] of T with
+                           // G = 16 / sizeof(T): entry (ct, g, L, i) = B[4 (G g + i) + L/16][16 ct + L%16],
+                           // zero past kpad, so lane L of a wave reads a column block's fragment in 16-byte loads
 };
 
 enum ScalarKind : int32_t {

@@ -631,19 +631,34 @@ __device__ __forceinline__ void mat_load_b(const XformDesc& x, int tid, int v0, 
       return;
     }
   }
+  // fragment-order copy of B: lane L's KS values of column block ct are G per
+  // 16-byte load, and a wave's load covers 1 KiB of consecutive bytes
+  constexpr int G = 16 / sizeof(T), NG = BFast<T>::KS / G;
   const int ctiles = x.npad / 16;
-  const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lk = lane >> 4;
+  const int wave = tid >> 6, lane = tid & 63;
   const MatPlan pl = mat_plan(ctiles, wave);
-  const T* __restrict__ B = (const T*)x.B;
+  const sod_u4* __restrict__ Bf = (const sod_u4*)x.Bf;
 #pragma unroll
   for (int u = 0; u < BFast<T>::TPW; ++u) {
     const int ct = pl.col0 + pl.cstep * (v0 + u);
     const bool has = ct < ctiles;
-    const T* bcol = B + (has ? ct : 0) * 16 + lr;
+    const sod_u4* src = Bf + (has ? ct : 0) * (NG * 64) + lane;
 #pragma unroll
-    for (int c = 0; c < BFast<T>::KS; ++c) {
-      const int k = 4 * c + lk;
-      br.b[u][c] = (has && k < x.kpad) ? gld(&bcol[(int64_t)k * x.npad]) : (T)0;
+    for (int g = 0; g < NG; ++g) {
+      sod_u4 v = {0u, 0u, 0u, 0u};
+      if (has) v = gld(&src[g * 64]);
+      if constexpr (G == 4) {
+        // through a scalar: clang's __builtin_bit_cast of a vector element
+        // expression (v[i]) reads element 0 for every i
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const unsigned int e = v[i];
+          br.b[u][4 * g + i] = __builtin_bit_cast(T, e);
+        }
+      } else {
+        br.b[u][2 * g] = __builtin_bit_cast(T, ((uint64_t)v[1] << 32) | v[0]);
+        br.b[u][2 * g + 1] = __builtin_bit_cast(T, ((uint64_t)v[3] << 32) | v[2]);
+      }
     }
   }
 }

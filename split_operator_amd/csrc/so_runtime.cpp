@@ -835,7 +835,9 @@ static void flush_op(so_op* op) {
 }
 
 // GEMM operand for a matrix transformation: B = M^T (real) or the real 2n x 2n
-// form of M^T (complex), zero-padded to [kpad][npad].
+// form of M^T (complex), zero-padded to [kpad][npad]. For kpad <= 128 the same
+// allocation continues with B in MFMA-fragment order (XformDesc::Bf), at
+// gemm_operand_frag().
 static void* gemm_operand(so_op* op, int which, int dtype, int* cplx_out) {
   if (op->gemm_b[which][dtype]) {
     *cplx_out = op->gemm_cplx[which];
@@ -862,13 +864,29 @@ static void* gemm_operand(so_op* op, int which, int dtype, int* cplx_out) {
         B[(2 * k + 1) * npad + 2 * r + 1] = mr;
       }
     }
+  // fragment order: G values of one 16-byte load per lane, entry (ct, g, L, i)
+  // = B[4 (G g + i) + L / 16][16 ct + L % 16] (the k of MFMA step 4 (G g + i)
+  // that lane L supplies for column block ct)
+  const int64_t G = dtype == SO_COMPLEX_F64 ? 2 : 4;
+  std::vector<double> all(B);
+  if (kpad <= 128) {
+    all.resize(B.size() + (size_t)(npad * 128), 0.0);
+    double* f = all.data() + B.size();
+    for (int64_t ct = 0; ct < npad / 16; ++ct)
+      for (int64_t g = 0; g < 128 / (4 * G); ++g)
+        for (int64_t L = 0; L < 64; ++L)
+          for (int64_t i = 0; i < G; ++i) {
+            const int64_t k = 4 * (G * g + i) + L / 16, col = 16 * ct + L % 16;
+            f[((ct * (128 / (4 * G)) + g) * 64 + L) * G + i] = k < kpad ? B[k * npad + col] : 0.0;
+          }
+  }
   void* d;
   if (dtype == SO_COMPLEX_F64) {
-    d = op->ctx->alloc(sizeof(double) * B.size());
-    HIPCHK(hipMemcpy(d, B.data(), sizeof(double) * B.size(), hipMemcpyHostToDevice));
+    d = op->ctx->alloc(sizeof(double) * all.size());
+    HIPCHK(hipMemcpy(d, all.data(), sizeof(double) * all.size(), hipMemcpyHostToDevice));
   } else {
-    std::vector<float> f(B.size());
-    for (size_t i = 0; i < B.size(); ++i) f[i] = (float)B[i];
+    std::vector<float> f(all.size());
+    for (size_t i = 0; i < all.size(); ++i) f[i] = (float)all[i];
     d = op->ctx->alloc(sizeof(float) * f.size());
     HIPCHK(hipMemcpy(d, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
   }
@@ -876,6 +894,13 @@ static void* gemm_operand(so_op* op, int which, int dtype, int* cplx_out) {
   op->gemm_cplx[which] = cplx ? 1 : 0;
   *cplx_out = cplx ? 1 : 0;
   return d;
+}
+
+// The fragment-order copy of gemm_operand's B (null for kpad > 128).
+static const void* gemm_operand_frag(const void* B, int dtype, int64_t kpad, int64_t npad) {
+  if (kpad > 128) return nullptr;
+  const size_t es = dtype == SO_COMPLEX_F64 ? sizeof(double) : sizeof(float);
+  return (const char*)B + es * (size_t)(kpad * npad);
 }
 
 static uint16_t bf16_rne_host(float f) {
@@ -1053,6 +1078,7 @@ std::unique_ptr<CompiledPass> Propagation::build(const PassB& b, bool immediate,
         d.kpad = (d.kdim + 3) / 4 * 4;
         d.ncols = d.kdim;
         d.npad = (d.ncols + 15) / 16 * 16;
+        d.Bf = gemm_operand_frag(d.B, arith_dt(dt), d.kpad, d.npad);
         if (gemm_mode == SO_GEMM_SPLIT_BF16X3 && arith_dt(dt) == 0 && d.kpad <= 128) {
           void *bh = nullptr, *bl = nullptr;
           gemm_operand_bf16(x.op, x.which, &bh, &bl);
@@ -2158,6 +2184,11 @@ extern "C" int so_matrix_transformation_set_matrix(so_op* t, int64_t n, const do
       t->ctx->release(t->gemm_b[w][d]);
       t->gemm_b[w][d] = nullptr;
     }
+  for (int w = 0; w < 2; ++w) {
+    t->ctx->release(t->gemm_bh[w]);
+    t->ctx->release(t->gemm_bl[w]);
+    t->gemm_bh[w] = t->gemm_bl[w] = nullptr;
+  }
   API_END
 }
 
